@@ -1,0 +1,219 @@
+/*
+ * watercube.h -- C ABI of the MI355X (gfx950) SPH step library.
+ *
+ * The library replaces the GPU half of WaterCube's per-frame simulation: the grid build of
+ * src/core/Sort.{h,cpp} (cell hash, per-cell count, prefix-sum offsets, reorder) and the two
+ * fluid passes that src/core/Fluid.cpp dispatches (density/pressure, force + integration with
+ * the cube boundary).  Every entry point below names the reference interface it stands in for
+ * as file:line relative to the reference repository root.
+ *
+ * Conventions
+ *   - Every call returns an int status: WC_OK (0) or a negative WC_ERR_* code.  The text of the
+ *     last failure on the calling thread is returned by wc_last_error().  Nothing throws across
+ *     the ABI.
+ *   - A context owns all device memory it allocates, for its whole lifetime, and one HIP stream.
+ *     Calls on one context are ordered on that stream; a context is not thread-safe.
+ *   - Host arrays belong to the caller.  Sizes are element counts.
+ *   - The particle record is the reference's 32-byte std430 struct, byte for byte.
+ */
+#ifndef WATERCUBE_H
+#define WATERCUBE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WC_API __attribute__((visibility("default")))
+
+#define WC_ABI_VERSION 1
+
+/* Status codes. */
+#define WC_OK 0
+#define WC_ERR_INVALID_ARG (-1) /* bad pointer, size or parameter value */
+#define WC_ERR_HIP (-2)         /* a HIP runtime call failed; text names the call */
+#define WC_ERR_NO_MEMORY (-3)   /* device or host allocation failed */
+#define WC_ERR_NO_DEVICE (-4)   /* no usable gfx950 device */
+#define WC_ERR_CAPACITY (-5)    /* size above what the context was created for */
+#define WC_ERR_DEVICE_FAULT (-6) /* a kernel reported a failed bounded wait (scan look-back) */
+
+/*
+ * One particle: struct Particle of src/core/util.h:29-35, laid out as the std430 GLSL struct
+ * of assets/sort/count.comp:5-10 (vec3 + float, vec3 + float): offsets 0, 12, 16, 28; 32 bytes.
+ */
+typedef struct wc_particle {
+    float position[3];
+    float density;
+    float velocity[3];
+    float pressure;
+} wc_particle;
+
+/* Equation of state used by the density pass. */
+#define WC_EOS_CUBIC 0  /* p = p0 + k((rho/rho0)^3 - 1): assets/fluid/density.comp:132-133 (live) */
+#define WC_EOS_LINEAR 1 /* p = p0 + k(rho - rho0): assets/fluid/density.comp:129 (commented alternative) */
+
+/*
+ * Simulation parameters: the uniforms the reference passes per dispatch
+ * (src/core/Sort.cpp:173-175, src/core/Fluid.cpp:276-285 and 305-317) plus the constants its
+ * shaders hard-code (assets/fluid/update.comp:5, 202-203), each made a field.
+ * wc_params_derive() fills bin_size and the three kernel constants from the primary fields.
+ */
+typedef struct wc_params {
+    uint32_t num_particles; /* num_particles_ (src/core/Fluid.cpp:12) */
+    int32_t grid_res;       /* grid_res_ R (Fluid.cpp:14); the grid has R^3 cells */
+    float size;             /* size_: the cube is [0,size]^3 (Fluid.cpp:10) */
+    float bin_size;         /* derived: size / R (Fluid.cpp:208) */
+    float kernel_radius;    /* h (Fluid.cpp:209) */
+    float particle_mass;    /* m (Fluid.cpp:210) */
+    float rest_density;     /* rho0 (Fluid.cpp:18) */
+    float rest_pressure;    /* p0 (Fluid.cpp:21) */
+    float stiffness;        /* k (Fluid.cpp:20) */
+    float viscosity;        /* mu, viscosityCoefficient (Fluid.cpp:19) */
+    float gravity[3];       /* gravity_direction_ * gravity_strength_ (Fluid.cpp:310) */
+    float poly6_const;      /* derived: (float)(315 / (64 pi h^9)) in double (Fluid.cpp:214) */
+    float spiky_const;      /* derived: (float)(-45 / (pi h^6)) (Fluid.cpp:215) */
+    float visc_const;       /* derived: (float)(45 / (pi h^6)) (Fluid.cpp:216) */
+    float wall_damping;     /* update.comp:202 (0.3 in the reference) */
+    float border;           /* update.comp:203 (0.001) */
+    float max_speed;        /* update.comp:5 MAX_SPEED (50) */
+    int32_t eos;            /* WC_EOS_CUBIC or WC_EOS_LINEAR */
+    int32_t wall_density;   /* 1: add density.comp:57-79 wallDensity (with its z-branch test of p.y) */
+    int32_t wall_force;     /* 1: add update.comp:71-100 wallForces (with vec3.length() == 3) */
+    int32_t mouse_force;    /* 1: add update.comp:105-132 mouseForce for the ray below */
+    float camera_position[3];     /* mouse ray origin (Fluid.cpp:314) */
+    float mouse_ray_direction[3]; /* mouse ray direction (Fluid.cpp:315) */
+    float time_scale;       /* time_scale_ (Fluid.cpp:24): dt = frame seconds * time_scale */
+    float particle_radius;  /* particle_radius_ (Fluid.cpp:17); the reference sets h = 4r, m = 8r */
+    uint32_t reserved[4];   /* zero */
+} wc_params;
+
+/* Parameter presets (wc_params_preset). */
+#define WC_PRESET_REFERENCE_DEFAULT 0 /* Fluid.cpp:9-27 defaults: 80,000 particles, R = 21, cubic EOS, wall terms on */
+#define WC_PRESET_DAM_BREAK_16K 1     /* 16,384 particles, h = 0.03, m = 3.375e-3, linear EOS k = 250, mu = 0.2 */
+#define WC_PRESET_BLOCK_1M 2          /* 1,000,000 particles, h = 0.016, m = 5.12e-4, R = 62 */
+#define WC_PRESET_BLOCK_SPHERE_17M 3  /* 17,777,216 particles, h = 0.004, m = 8e-6, R = 250 */
+
+/* Fill *p with a preset and derive its constants.  No reference counterpart beyond the
+ * constructor defaults of src/core/Fluid.cpp:9-27 and src/core/Fluid.cpp:206-216. */
+WC_API int wc_params_preset(wc_params* p, int preset);
+
+/* Recompute bin_size = size / grid_res and the poly6 / spiky / viscosity constants from
+ * kernel_radius, as Fluid::setup does (src/core/Fluid.cpp:206-216). */
+WC_API int wc_params_derive(wc_params* p);
+
+/* ------------------------------------------------------------------------------------------
+ * Scene generators (the step before the path: src/core/Fluid.cpp:104-134).  Host-only, no GPU.
+ * Random numbers come from SplitMix64 seeded by `seed`; each particle draws its jitter in the
+ * order x, y, z, each as u = (next() >> 40) * 2^-24, jitter = (2u - 1) * amplitude.
+ * ------------------------------------------------------------------------------------------ */
+
+/* Jittered lattice block: nx*ny*nz particles at origin + (i + 0.5) * spacing + jitter,
+ * x fastest, then y, then z (src/core/Fluid.cpp:116-131 walks the same order).  Velocity is
+ * `velocity` (may be NULL for zero); density and pressure are zero.  Writes nx*ny*nz records. */
+WC_API int wc_scene_block(wc_particle* out, uint32_t nx, uint32_t ny, uint32_t nz, const float origin[3],
+                          float spacing, float jitter, const float velocity[3], uint64_t seed);
+
+/* Jittered lattice sphere: the lattice points origin-free at (i + 0.5) * spacing inside the sphere
+ * of `center`, taken in x-fastest order, until `count` particles are written.  The radius is the
+ * smallest lattice-aligned one that holds `count` points.  Returns WC_ERR_INVALID_ARG if the
+ * sphere does not fit in [0,size]^3 for size = 1. */
+WC_API int wc_scene_sphere(wc_particle* out, uint32_t count, const float center[3], float spacing, float jitter,
+                           const float velocity[3], uint64_t seed);
+
+/* The reference's own initial state (src/core/Fluid.cpp:104-134): ceil(cbrt(n))^3 lattice at
+ * spacing 1.75 * particle_radius from the origin, jitter uniform in +-0.25 * spacing, first n
+ * points.  rand() is replaced by the SplitMix64 draw above. */
+WC_API int wc_scene_reference(wc_particle* out, uint32_t n, float particle_radius, uint64_t seed);
+
+/* ------------------------------------------------------------------------------------------
+ * Simulation context: Fluid's particle buffers + Sort's grid buffers on one GPU.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct wc_ctx wc_ctx;
+
+/* Allocate device buffers for p->num_particles particles and grid_res^3 cells on the current
+ * HIP device, create the context's stream.  Stands in for Fluid::setup's buffer half and
+ * Sort::prepareBuffers (src/core/Fluid.cpp:203-235, 139-166; src/core/Sort.cpp:67-94).  Unlike
+ * the reference (Sort.cpp:76-82 sizes them by particle count) counts and offsets hold R^3 cells. */
+WC_API int wc_create(const wc_params* p, wc_ctx** out);
+
+/* Free everything the context owns.  NULL is accepted. */
+WC_API int wc_destroy(wc_ctx* ctx);
+
+/* Replace the parameters; num_particles and grid_res must not exceed the creation values. */
+WC_API int wc_set_params(wc_ctx* ctx, const wc_params* p);
+WC_API int wc_get_params(const wc_ctx* ctx, wc_params* out);
+
+/* Copy n records into particle buffer 1 (the buffer Fluid::update sorts from and the update
+ * pass writes to: Fluid.cpp:146, 347-350).  Both buffers are filled, like Fluid.cpp:146 and 155. */
+WC_API int wc_upload(wc_ctx* ctx, const wc_particle* host, uint32_t n);
+
+/* Copy n records out of particle buffer `which` (1 or 2); waits for the stream.  Stands in for
+ * util::getParticles (src/core/util.cpp:42-57). */
+WC_API int wc_download(wc_ctx* ctx, int which, wc_particle* host, uint32_t n);
+
+/* Sort::run(buffer1, buffer2) (src/core/Sort.cpp:254-267): clear counts, count, exclusive
+ * scan, reorder -- as count.comp:25-36, linearScan.comp:15-28, reorder.comp:32-45 -- but with a
+ * stable within-cell order (particles keep their buffer-1 order inside a cell). */
+WC_API int wc_sort_run(wc_ctx* ctx);
+
+/* Fluid::runDensityProg(buffer2) (src/core/Fluid.cpp:268-289): density.comp:81-137 in place. */
+WC_API int wc_density(wc_ctx* ctx);
+
+/* Fluid::runUpdateProg(buffer2, buffer1, dt) (src/core/Fluid.cpp:294-321): update.comp:134-232.
+ * `dt` is the integration step itself (the reference passes frame time * time_scale_). */
+WC_API int wc_update(wc_ctx* ctx, float dt);
+
+/* nsteps of Fluid::update (src/core/Fluid.cpp:342-354): sort, density, update, replayed as one
+ * captured HIP graph per step.  Asynchronous: returns once the work is queued. */
+WC_API int wc_step(wc_ctx* ctx, float dt, uint32_t nsteps);
+
+/* Wait for the context's stream; reports a device-side failure flag as WC_ERR_DEVICE_FAULT. */
+WC_API int wc_synchronize(wc_ctx* ctx);
+
+/* Per-cell particle counts and exclusive offsets after the last sort (R^3 entries each): the
+ * contents of Sort::getCountBuffer / getOffsetBuffer (src/core/Sort.h:35-36). */
+WC_API int wc_get_counts(wc_ctx* ctx, uint32_t* host, uint32_t ncells);
+WC_API int wc_get_offsets(wc_ctx* ctx, uint32_t* host, uint32_t ncells);
+
+/* Cell index of every particle of buffer 1, as count.comp:31-33 computes it (n entries). */
+WC_API int wc_get_cell_keys(wc_ctx* ctx, uint32_t* host, uint32_t n);
+
+/* Sorted order of the last sort: out[slot] = index in buffer 1 of the particle now at `slot` of
+ * buffer 2.  The permutation sort.comp:32-45 writes into Sort::getSortedBuffer (Sort.h:37). */
+WC_API int wc_get_sorted(wc_ctx* ctx, uint32_t* host, uint32_t n);
+
+/* Device addresses of the context's buffers, for callers that keep data in HBM.
+ * posrho[b] / velp[b] are float4 arrays (x, y, z, density) and (vx, vy, vz, pressure) of
+ * particle buffer b+1; cell_range is uint2 (offset, count) per cell. */
+typedef struct wc_device_view {
+    void* posrho[2];
+    void* velp[2];
+    void* cell_range;
+    void* sorted;
+    void* stream; /* hipStream_t */
+    uint32_t num_particles;
+    uint32_t num_cells;
+} wc_device_view;
+WC_API int wc_device_view_get(wc_ctx* ctx, wc_device_view* out);
+
+/* Timing of each kernel of the step, measured with HIP events recorded on the context's stream
+ * around every launch (eager launches, no graph).  Runs nsteps full steps and returns in
+ * ms_total[k] the summed duration of kernel k over them; wc_pass_name(k) names kernel k. */
+#define WC_NUM_PASSES 6
+WC_API int wc_profile_steps(wc_ctx* ctx, float dt, uint32_t nsteps, double ms_total[WC_NUM_PASSES]);
+WC_API const char* wc_pass_name(int k);
+
+/* Text of the last failure on the calling thread ("" if none). */
+WC_API const char* wc_last_error(void);
+
+/* ABI version (WC_ABI_VERSION) and the device the library targets ("gfx950"). */
+WC_API int wc_abi_version(void);
+WC_API const char* wc_target_arch(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WATERCUBE_H */

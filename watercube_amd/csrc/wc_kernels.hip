@@ -1,0 +1,544 @@
+// wc_kernels.hip -- gfx950 kernels of the SPH step: grid build (hash + histogram, decoupled
+// look-back scan, place, stable within-cell rank + gather), density/pressure, force + integrate.
+//
+// Floating-point contract (shared with the CPU transcription): binary32, correctly rounded '/'
+// and sqrtf (-fhip-fp32-correctly-rounded-divide-sqrt), subnormals kept, no FMA contraction
+// (-ffp-contract=off), every expression associated left to right as the GLSL source writes it.
+#include <hip/hip_runtime.h>
+
+#include "wc_internal.hpp"
+
+namespace wc {
+namespace {
+
+__device__ __forceinline__ int cell_axis(float x, float bin, int R) {
+    // ivec3(p / binSize) truncates toward zero, then clamp(.., 0, R-1) (count.comp:32).
+    const float q = x / bin;
+    if (!(q >= 0.0f)) return 0;
+    if (q >= (float)R) return R - 1;
+    const int c = (int)q;
+    return c > R - 1 ? R - 1 : c;
+}
+
+__device__ __forceinline__ uint32_t cell_key(const StepConsts& c, float x, float y, float z) {
+    const int cx = cell_axis(x, c.bin, c.R);
+    const int cy = cell_axis(y, c.bin, c.R);
+    const int cz = cell_axis(z, c.bin, c.R);
+    return (uint32_t)((cz * c.R + cy) * c.R + cx);
+}
+
+__device__ __forceinline__ float poly6(const StepConsts& c, float r) {
+    const float a = c.hh - r * r;
+    return ((a * a) * a) * c.poly6;
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// ------------------------------------------------------------------------------------------
+// 1. cell hash + histogram (count.comp:25-36).  Consecutive lanes that fall in the same cell
+//    form one run; the run's first lane does a single atomicAdd for the whole run, every lane
+//    keeps its arrival rank.  Also resets the scan's look-back state for this step.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ posrho, uint32_t* __restrict__ key,
+                                                 uint32_t* __restrict__ rank, uint32_t* __restrict__ counts,
+                                                 unsigned long long* __restrict__ scan_status,
+                                                 uint32_t* __restrict__ scan_ticket, uint32_t num_tiles,
+                                                 StepConsts c) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < num_tiles) scan_status[i] = 0ull;
+    if (i == 0) *scan_ticket = 0u;
+
+    const bool valid = i < c.n;
+    uint32_t k = kNoKey;
+    if (valid) {
+        const float4 p = posrho[i];
+        k = cell_key(c, p.x, p.y, p.z);
+    }
+    const unsigned lane = lane_id();
+    const uint32_t prev = __shfl_up(k, 1);
+    const bool leader = (lane == 0) || (prev != k);
+    const unsigned long long leaders = __ballot(leader);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+    const int first = 63 - __clzll((long long)(leaders & upto));
+    const unsigned long long above = leaders & ~upto;
+    const int next = above ? (__ffsll((long long)above) - 1) : 64;
+    uint32_t base = 0;
+    if (leader && valid) base = atomicAdd(&counts[k], (uint32_t)(next - first));
+    base = __shfl(base, first);
+    if (valid) {
+        key[i] = k;
+        rank[i] = base + (uint32_t)((int)lane - first);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// 2. exclusive scan of the per-cell counts (linearScan.comp:15-28), single pass with decoupled
+//    look-back.  Writes range[cell] = (offset, count) and zeroes counts for the next step.
+//    Look-back word: (flag << 32) | value; flag 1 = tile aggregate, 2 = inclusive prefix.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long ld_status(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ counts, uint2* __restrict__ range,
+                                                       unsigned long long* __restrict__ status,
+                                                       uint32_t* __restrict__ ticket, uint32_t ncells,
+                                                       uint32_t* __restrict__ error_flag) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint32_t s_prefix;
+    const unsigned tid = threadIdx.x;
+    const unsigned lane = lane_id();
+    const unsigned wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t base = (size_t)tile * kScanTile + (size_t)tid * kScanItems;
+
+    uint32_t v[kScanItems];
+    if (base + kScanItems <= ncells) {
+        const uint4* src = reinterpret_cast<const uint4*>(counts + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const uint4 w = src[q];
+            v[4 * q + 0] = w.x;
+            v[4 * q + 1] = w.y;
+            v[4 * q + 2] = w.z;
+            v[4 * q + 3] = w.w;
+        }
+        uint4* dst = reinterpret_cast<uint4*>(counts + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) dst[q] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            v[j] = (base + j < ncells) ? counts[base + j] : 0u;
+            if (base + j < ncells) counts[base + j] = 0u;
+        }
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) sum += v[j];
+
+    uint32_t incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d);
+        if ((int)lane >= d) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_prefix = 0, tile_total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        const uint32_t t = s_wave[w];
+        if ((unsigned)w < wave) wave_prefix += t;
+        tile_total += t;
+    }
+    const uint32_t thread_excl = wave_prefix + incl - sum;
+
+    if (wave == 0) {
+        uint32_t tile_prefix = 0;
+        if (tile == 0) {
+            if (lane == 0) st_status(&status[0], (2ull << 32) | tile_total);
+        } else {
+            if (lane == 0) st_status(&status[tile], (1ull << 32) | tile_total);
+            long long look = (long long)tile - 1;
+            uint32_t acc = 0, spins = 0;
+            for (;;) {
+                const long long idx = look - (long long)lane;
+                unsigned long long s = (idx >= 0) ? ld_status(&status[idx]) : (2ull << 32);
+                const uint32_t flag = (uint32_t)(s >> 32);
+                const unsigned long long not_ready = __ballot(flag == 0u);
+                const unsigned long long is_prefix = __ballot(flag == 2u);
+                const int first_prefix = is_prefix ? (__ffsll((long long)is_prefix) - 1) : 64;
+                const unsigned long long window =
+                    (first_prefix >= 63) ? ~0ull : ((2ull << first_prefix) - 1ull);
+                if (not_ready & window) {
+                    if (++spins > kLookbackSpinLimit) {
+                        if (lane == 0) atomicOr(error_flag, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                const uint32_t contrib = ((int)lane <= first_prefix) ? (uint32_t)s : 0u;
+                acc += wave_sum(contrib);
+                if (first_prefix < 64) break;
+                look -= 64;
+            }
+            tile_prefix = acc;
+            if (lane == 0) st_status(&status[tile], (2ull << 32) | (uint32_t)(acc + tile_total));
+        }
+        if (lane == 0) s_prefix = tile_prefix;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + thread_excl;
+    if (base + kScanItems <= ncells) {
+        uint4* dst = reinterpret_cast<uint4*>(range + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 2; ++q) {
+            const uint32_t a = run;
+            run += v[2 * q];
+            const uint32_t b2 = run;
+            run += v[2 * q + 1];
+            dst[q] = make_uint4(a, v[2 * q], b2, v[2 * q + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            if (base + j < ncells) range[base + j] = make_uint2(run, v[j]);
+            run += v[j];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// 3. place: slot = offset[cell] + arrival rank (reorder.comp:38-43 without the record copy).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
+                                                  const uint2* __restrict__ range, uint32_t* __restrict__ slot_tmp,
+                                                  uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = key[i];
+    const uint32_t slot = range[k].x + rank[i];
+    if (slot < n) slot_tmp[slot] = i;  // always true when the scan succeeded
+}
+
+// ------------------------------------------------------------------------------------------
+// 4. stable rank + gather: the particle that arrived at slot t moves to offset + (number of
+//    particles of its cell with a smaller input index), so every cell keeps input order; the
+//    record is copied there (reorder.comp:44) and the cell index kept beside it.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ slot_tmp,
+                                                   const uint32_t* __restrict__ key, const uint2* __restrict__ range,
+                                                   const float4* __restrict__ posrho_in,
+                                                   const float4* __restrict__ velp_in, float4* __restrict__ posrho_out,
+                                                   float4* __restrict__ velp_out, uint32_t* __restrict__ sorted,
+                                                   uint32_t* __restrict__ skey, uint32_t n) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = slot_tmp[t];
+    if (i >= n) return;  // only after a failed scan (error flag set)
+    const uint32_t k = key[i];
+    const uint2 rg = range[k];
+    const uint32_t end = min(rg.x + rg.y, n);
+    uint32_t r = 0;
+    for (uint32_t u = rg.x; u < end; ++u) r += (slot_tmp[u] < i) ? 1u : 0u;
+    const uint32_t dst = rg.x + r;
+    if (dst >= n) return;
+    sorted[dst] = i;
+    skey[dst] = k;
+    posrho_out[dst] = posrho_in[i];
+    velp_out[dst] = velp_in[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// 5. density + pressure (density.comp:81-137), in place on the sorted buffer.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float wall_density(const StepConsts& c, float x, float y, float z) {
+    // density.comp:57-79; the z branch's else-test reads p.y (line 74)
+    float d = 0.0f;
+    if (x < c.h) d += c.m * poly6(c, x);
+    else if (x > c.size_minus_h) d += c.m * poly6(c, c.size - x);
+    if (y < c.h) d += c.m * poly6(c, y);
+    else if (y > c.size_minus_h) d += c.m * poly6(c, c.size - y);
+    if (z < c.h) d += c.m * poly6(c, z);
+    else if (y > c.size_minus_h) d += c.m * poly6(c, c.size - z);
+    return d * 4.0f;
+}
+
+__device__ __forceinline__ float eos(const StepConsts& c, float rho) {
+    if (c.eos == WC_EOS_LINEAR) return c.p0 + c.k * (rho - c.rho0);
+    const float x = rho / c.rho0;
+    return c.p0 + c.k * (((x * x) * x) - 1.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_density(float4* __restrict__ posrho, float4* __restrict__ velp,
+                                                    const uint32_t* __restrict__ skey,
+                                                    const uint2* __restrict__ range, StepConsts c) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= c.n) return;
+    const float4 p = posrho[t];
+    const uint32_t key = skey[t];
+    const int R = c.R;
+    const int cx = (int)(key % (uint32_t)R);
+    const int cy = (int)((key / (uint32_t)R) % (uint32_t)R);
+    const int cz = (int)(key / ((uint32_t)R * (uint32_t)R));
+
+    float rho = c.m * poly6(c, 0.0f);
+#pragma unroll 1
+    for (int b = 0; b < 27; ++b) {
+        const int nx = cx + (b / 9 - 1);
+        const int ny = cy + ((b / 3) % 3 - 1);
+        const int nz = cz + (b % 3 - 1);
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= R || ny >= R || nz >= R) continue;
+        const uint2 rg = range[(nz * R + ny) * R + nx];
+        const uint32_t end = min(rg.x + rg.y, c.n);
+        for (uint32_t u = rg.x; u < end; ++u) {
+            if (u == t) continue;
+            const float4 q = posrho[u];
+            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+            const float d2 = (rx * rx + ry * ry) + rz * rz;
+            if (d2 >= c.cut2) continue;  // same outcome as sqrtf(d2) >= h
+            const float dist = sqrtf(d2);
+            rho += c.m * poly6(c, dist);
+        }
+    }
+    const float wall = c.wall_density ? wall_density(c, p.x, p.y, p.z) : 0.0f;
+    reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = rho + wall;
+    reinterpret_cast<float*>(velp)[4 * (size_t)t + 3] = eos(c, rho);
+}
+
+// ------------------------------------------------------------------------------------------
+// 6. forces + symplectic Euler + cube boundary (update.comp:134-232): sorted buffer -> buffer 1.
+// ------------------------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 spiky(const StepConsts& c, float rx, float ry, float rz, float d) {
+    const float a = c.h - d;
+    const float s = a * a;
+    return V3{(s * (rx / d)) * c.spiky, (s * (ry / d)) * c.spiky, (s * (rz / d)) * c.spiky};
+}
+
+__device__ __forceinline__ V3 wall_forces(const StepConsts& c, float x, float y, float z) {
+    // update.comp:71-100; vec3.length() is 3
+    V3 f{0.0f, 0.0f, 0.0f};
+    if (x < c.h) {
+        const V3 w = spiky(c, 0.0f - x, y - y, z - z, 3.0f);
+        f.x += w.x; f.y += w.y; f.z += w.z;
+    } else if (x > c.size_minus_h) {
+        const V3 w = spiky(c, c.size - x, y - y, z - z, 3.0f);
+        f.x += w.x; f.y += w.y; f.z += w.z;
+    }
+    if (y < c.h) {
+        const V3 w = spiky(c, x - x, 0.0f - y, z - z, 3.0f);
+        f.x += w.x; f.y += w.y; f.z += w.z;
+    } else if (y > c.size_minus_h) {
+        const V3 w = spiky(c, x - x, c.size - y, z - z, 3.0f);
+        f.x += w.x; f.y += w.y; f.z += w.z;
+    }
+    if (z < c.h) {
+        const V3 w = spiky(c, x - x, y - y, 0.0f - z, 3.0f);
+        f.x += w.x; f.y += w.y; f.z += w.z;
+    } else if (z > c.size_minus_h) {
+        const V3 w = spiky(c, x - x, y - y, c.size - z, 3.0f);
+        f.x += w.x; f.y += w.y; f.z += w.z;
+    }
+    return V3{f.x * 0.01f, f.y * 0.01f, f.z * 0.01f};
+}
+
+__device__ __forceinline__ V3 mouse_force(const StepConsts& c, float px, float py, float pz, float pressure) {
+    // update.comp:105-132
+    const float o[3] = {c.cam[0], c.cam[1], c.cam[2]};
+    float t1[3], t2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float tmin = (0.0f - o[a]) / c.dir[a];
+        const float tmax = (c.size - o[a]) / c.dir[a];
+        t1[a] = fminf(tmin, tmax);
+        t2[a] = fmaxf(tmin, tmax);
+    }
+    const float tnear = fmaxf(fmaxf(t1[0], t1[1]), t1[2]);
+    const float tfar = fminf(fminf(t2[0], t2[1]), t2[2]);
+    if (tnear > tfar) return V3{0.0f, 0.0f, 0.0f};
+    const float tx = px - o[0], ty = py - o[1], tz = pz - o[2];
+    const float cx = c.dir[1] * tz - ty * c.dir[2];
+    const float cy = c.dir[2] * tx - tz * c.dir[0];
+    const float cz = c.dir[0] * ty - tx * c.dir[1];
+    const float dist = sqrtf((cx * cx + cy * cy) + cz * cz);
+    if (dist > c.h) return V3{0.0f, 0.0f, 0.0f};
+    const V3 w = spiky(c, tx, ty, tz, dist + 1e-16f);
+    const float s = (-c.m) * pressure;
+    return V3{(s * w.x) * 0.00001f, (s * w.y) * 0.00001f, (s * w.z) * 0.00001f};
+}
+
+__device__ __forceinline__ void integrate_axis(const StepConsts& c, float pf, float vf, float ext, float denom,
+                                               float v0, float x0, float dt, float& vel, float& pos) {
+    vf = vf * c.mu;
+    const float force = (pf + vf) + ext;
+    const float acc = force / denom;
+    const float v = v0 + acc * dt;
+    vel = fminf(fmaxf(v, -c.max_speed), c.max_speed);
+    pos = x0 + vel * dt;
+    if (pos < c.border) {
+        vel = vel * -c.damping;
+        pos = c.border;
+    } else if (pos > c.hi) {
+        vel = vel * -c.damping;
+        pos = c.hi;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_update(const float4* __restrict__ posrho_in,
+                                                   const float4* __restrict__ velp_in,
+                                                   const uint32_t* __restrict__ skey,
+                                                   const uint2* __restrict__ range, float4* __restrict__ posrho_out,
+                                                   float4* __restrict__ velp_out, StepConsts c, float dt) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= c.n) return;
+    const float4 p = posrho_in[t];
+    const float4 pv = velp_in[t];
+    const uint32_t key = skey[t];
+    const int R = c.R;
+    const int cx = (int)(key % (uint32_t)R);
+    const int cy = (int)((key / (uint32_t)R) % (uint32_t)R);
+    const int cz = (int)(key / ((uint32_t)R * (uint32_t)R));
+
+    float pfx = 0.0f, pfy = 0.0f, pfz = 0.0f;
+    float vfx = 0.0f, vfy = 0.0f, vfz = 0.0f;
+#pragma unroll 1
+    for (int b = 0; b < 27; ++b) {
+        const int nx = cx + (b / 9 - 1);
+        const int ny = cy + ((b / 3) % 3 - 1);
+        const int nz = cz + (b % 3 - 1);
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= R || ny >= R || nz >= R) continue;
+        const uint2 rg = range[(nz * R + ny) * R + nx];
+        const uint32_t end = min(rg.x + rg.y, c.n);
+        for (uint32_t u = rg.x; u < end; ++u) {
+            if (u == t) continue;
+            const float4 q = posrho_in[u];
+            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+            const float d2 = (rx * rx + ry * ry) + rz * rz;
+            if (d2 >= c.cut2) continue;
+            const float dist = sqrtf(d2);
+            const float4 qv = velp_in[u];
+            const float pressure = (pv.w + qv.w) / (2.0f * q.w);
+            if (pressure > 0.0f) {
+                const V3 w = spiky(c, rx, ry, rz, dist + 1e-16f);
+                const float mp = c.m * pressure;
+                pfx = pfx - mp * w.x;
+                pfy = pfy - mp * w.y;
+                pfz = pfz - mp * w.z;
+            }
+            const float wv = (c.h - dist) * c.visc;
+            vfx = vfx + (c.m * ((qv.x - pv.x) / q.w)) * wv;
+            vfy = vfy + (c.m * ((qv.y - pv.y) / q.w)) * wv;
+            vfz = vfz + (c.m * ((qv.z - pv.z) / q.w)) * wv;
+        }
+    }
+    V3 ext{c.g[0] * p.w, c.g[1] * p.w, c.g[2] * p.w};
+    V3 mouse{0.0f, 0.0f, 0.0f}, wall{0.0f, 0.0f, 0.0f};
+    if (c.mouse_force) mouse = mouse_force(c, p.x, p.y, p.z, pv.w);
+    if (c.wall_force) wall = wall_forces(c, p.x, p.y, p.z);
+    ext.x = ext.x + (mouse.x + wall.x);
+    ext.y = ext.y + (mouse.y + wall.y);
+    ext.z = ext.z + (mouse.z + wall.z);
+
+    const float denom = p.w + 1e-16f;
+    float4 po, vo;
+    integrate_axis(c, pfx, vfx, ext.x, denom, pv.x, p.x, dt, vo.x, po.x);
+    integrate_axis(c, pfy, vfy, ext.y, denom, pv.y, p.y, dt, vo.y, po.y);
+    integrate_axis(c, pfz, vfz, ext.z, denom, pv.z, p.z, dt, vo.z, po.z);
+    po.w = p.w;
+    vo.w = pv.w;
+    posrho_out[t] = po;
+    velp_out[t] = vo;
+}
+
+// ------------------------------------------------------------------------------------------
+// Record <-> split-half conversion and fills.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_split(const float4* __restrict__ rec, float4* __restrict__ posrho,
+                                                  float4* __restrict__ velp, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    posrho[i] = rec[2 * (size_t)i];
+    velp[i] = rec[2 * (size_t)i + 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_join(const float4* __restrict__ posrho,
+                                                 const float4* __restrict__ velp, float4* __restrict__ rec,
+                                                 uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    rec[2 * (size_t)i] = posrho[i];
+    rec[2 * (size_t)i + 1] = velp[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+hipError_t launch_hash(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
+    const size_t work = c.n > b.num_scan_tiles ? c.n : b.num_scan_tiles;
+    if (work == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hash, dim3(blocks_for(work, kBlock)), dim3(kBlock), 0, s, b.posrho[0], b.key, b.rank,
+                       b.counts, b.scan_status, b.scan_ticket, b.num_scan_tiles, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
+    const unsigned tiles = blocks_for(c.ncells, kScanTile);
+    if (tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(kScanThreads), 0, s, b.counts, b.range, b.scan_status,
+                       b.scan_ticket, c.ncells, b.error_flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_place(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
+    if (c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_place, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.key, b.rank, b.range,
+                       b.slot_tmp, c.n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
+    if (c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.slot_tmp, b.key, b.range,
+                       b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, c.n);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
+    if (c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_density, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.posrho[1], b.velp[1],
+                       b.skey, b.range, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, hipStream_t s) {
+    if (c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_update, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.posrho[1], b.velp[1],
+                       b.skey, b.range, b.posrho[0], b.velp[0], c, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_split, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                       reinterpret_cast<const float4*>(rec), posrho, velp, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,
+                               hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_join, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, posrho, velp,
+                       reinterpret_cast<float4*>(rec), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+}  // namespace wc
