@@ -124,6 +124,7 @@ int free_ctx_buffers(wc_ctx* c) {
     DeviceBuffers& b = c->b;
     void* ptrs[] = {b.posrho[0], b.posrho[1], b.velp[0], b.velp[1], b.key,         b.rank,
                     b.slot_tmp,  b.sorted,    b.skey,    b.counts,  b.range,       b.scan_status,
+                    b.nlist,     b.ncount,
                     b.scan_ticket, b.error_flag, c->staging};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -280,6 +281,8 @@ WC_API int wc_create(const wc_params* p, wc_ctx** out) {
     alloc((void**)&b.skey, n * 4);
     alloc((void**)&b.counts, cells * 4 + 64);
     alloc((void**)&b.range, cells * 8 + 64);
+    alloc((void**)&b.nlist, n * 4 * wc::kMaxNeighbours);
+    alloc((void**)&b.ncount, n * 4);
     alloc((void**)&b.scan_status, (size_t)tiles * 8 + 8);
     alloc((void**)&b.scan_ticket, 64);
     alloc((void**)&b.error_flag, 64);
@@ -297,6 +300,7 @@ WC_API int wc_create(const wc_params* p, wc_ctx** out) {
     if (e == hipSuccess) e = hipMemsetAsync(b.sorted, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.skey, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.key, 0, n * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b.ncount, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.posrho[0], 0, n * sizeof(float4), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.posrho[1], 0, n * sizeof(float4), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.velp[0], 0, n * sizeof(float4), c->stream);

@@ -14,6 +14,7 @@ constexpr int kScanItems = 16;        // counts per scan thread (four 16-byte lo
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 constexpr uint32_t kLookbackSpinLimit = 1u << 24;  // bounded wait in the scan look-back
+constexpr uint32_t kMaxNeighbours = 64;             // neighbour-list slots per particle
 
 // Everything one step's kernels read besides buffers, precomputed on the host in float so the
 // device sees exactly the bits the CPU transcription uses.
@@ -53,6 +54,8 @@ struct DeviceBuffers {
     uint32_t* skey;      // slot -> cell index of the sorted particle
     uint32_t* counts;    // R^3 atomic histogram; the scan reads and zeroes it
     uint2* range;        // R^3 (exclusive offset, count)
+    uint32_t* nlist;     // kMaxNeighbours x n: slot of the k-th accepted neighbour of slot t at k*n + t
+    uint32_t* ncount;    // n: accepted neighbours per slot (may exceed kMaxNeighbours)
     unsigned long long* scan_status;  // one look-back word per scan tile
     uint32_t* scan_ticket;            // dynamic tile counter
     uint32_t* error_flag;             // nonzero after a failed bounded wait

@@ -34,6 +34,41 @@ __device__ __forceinline__ float poly6(const StepConsts& c, float r) {
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
+// The dispatcher hands workgroup b to XCD b % 8.  Renumber workgroups so that every XCD works
+// through one contiguous run of logical blocks: neighbouring particles (which read the same
+// neighbour cells) then share one XCD's L2 instead of being fetched by all eight.
+constexpr uint32_t kNumXcd = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+    const uint32_t q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+    const uint32_t x = b % kNumXcd, k = b / kNumXcd;
+    return x * q + min(x, r) + k;
+}
+
+// Walks the in-grid neighbour cells of cell `key` in NEIGHBORHOOD order (density.comp:40-50:
+// dx outermost, dz innermost); visit(start, end) receives each cell's slot range and
+// end_plane() runs after every dx plane.  The cell loop is wave-uniform.
+template <typename Visit, typename EndPlane>
+__device__ __forceinline__ void for_each_neighbour_cell(uint32_t key, const uint2* __restrict__ range,
+                                                        const StepConsts& c, Visit& visit, EndPlane& end_plane) {
+    const int R = c.R;
+    const int cx = (int)(key % (uint32_t)R);
+    const int cy = (int)((key / (uint32_t)R) % (uint32_t)R);
+    const int cz = (int)(key / ((uint32_t)R * (uint32_t)R));
+#pragma unroll 1
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int nx = cx + dx;
+#pragma unroll 1
+        for (int dyz = 0; dyz < 9; ++dyz) {
+            const int ny = cy + (dyz / 3 - 1);
+            const int nz = cz + (dyz % 3 - 1);
+            if (nx < 0 || ny < 0 || nz < 0 || nx >= R || ny >= R || nz >= R) continue;
+            const uint2 rg = range[(nz * R + ny) * R + nx];
+            visit(rg.x, min(rg.x + rg.y, c.n));
+        }
+        end_plane();
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // 1. cell hash + histogram (count.comp:25-36).  Consecutive lanes that fall in the same cell
 //    form one run; the run's first lane does a single atomicAdd for the whole run, every lane
@@ -225,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ 
                                                    const float4* __restrict__ velp_in, float4* __restrict__ posrho_out,
                                                    float4* __restrict__ velp_out, uint32_t* __restrict__ sorted,
                                                    uint32_t* __restrict__ skey, uint32_t n) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = slot_tmp[t];
     if (i >= n) return;  // only after a failed scan (error flag set)
@@ -244,7 +279,20 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ 
 
 // ------------------------------------------------------------------------------------------
 // 5. density + pressure (density.comp:81-137), in place on the sorted buffer.
+//
+//    Neighbour search: each lane walks its own slot range of the current neighbour cell four
+//    slots at a time (four loads in flight).  Candidate tests are branch-free: every candidate
+//    is written to the lane's next LDS list entry and the entry count advances by the outcome.
+//    The list is drained after every dx plane (or when it could overflow): the density terms are
+//    then summed over compacted entries, with all lanes busy, in enumeration order.  The drain
+//    also appends each accepted neighbour's slot to the particle's neighbour list in HBM
+//    (nlist[k * n + t], k < kMaxNeighbours), which the force pass reads instead of searching
+//    again: positions do not change between the two passes, so the accepted set and its order
+//    are the same.
 // ------------------------------------------------------------------------------------------
+constexpr int kChunk = 4;
+constexpr int kListCap = 16;
+
 __device__ __forceinline__ float wall_density(const StepConsts& c, float x, float y, float z) {
     // density.comp:57-79; the z branch's else-test reads p.y (line 74)
     float d = 0.0f;
@@ -265,42 +313,60 @@ __device__ __forceinline__ float eos(const StepConsts& c, float rho) {
 
 __global__ __launch_bounds__(kBlock) void k_density(float4* __restrict__ posrho, float4* __restrict__ velp,
                                                     const uint32_t* __restrict__ skey,
-                                                    const uint2* __restrict__ range, StepConsts c) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+                                                    const uint2* __restrict__ range, uint32_t* __restrict__ nlist,
+                                                    uint32_t* __restrict__ ncount, StepConsts c) {
+    __shared__ float s_d2[kListCap * kBlock];
+    __shared__ uint32_t s_slot[kListCap * kBlock];
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= c.n) return;
     const float4 p = posrho[t];
-    const uint32_t key = skey[t];
-    const int R = c.R;
-    const int cx = (int)(key % (uint32_t)R);
-    const int cy = (int)((key / (uint32_t)R) % (uint32_t)R);
-    const int cz = (int)(key / ((uint32_t)R * (uint32_t)R));
+    float* l_d2 = s_d2 + threadIdx.x;
+    uint32_t* l_slot = s_slot + threadIdx.x;
 
     float rho = c.m * poly6(c, 0.0f);
-#pragma unroll 1
-    for (int b = 0; b < 27; ++b) {
-        const int nx = cx + (b / 9 - 1);
-        const int ny = cy + ((b / 3) % 3 - 1);
-        const int nz = cz + (b % 3 - 1);
-        if (nx < 0 || ny < 0 || nz < 0 || nx >= R || ny >= R || nz >= R) continue;
-        const uint2 rg = range[(nz * R + ny) * R + nx];
-        const uint32_t end = min(rg.x + rg.y, c.n);
-        for (uint32_t u = rg.x; u < end; ++u) {
-            if (u == t) continue;
-            const float4 q = posrho[u];
-            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-            const float d2 = (rx * rx + ry * ry) + rz * rz;
-            if (d2 >= c.cut2) continue;  // same outcome as sqrtf(d2) >= h
-            const float dist = sqrtf(d2);
+    uint32_t total = 0;
+    int cnt = 0;
+    auto drain = [&]() {
+        for (int k = 0; k < cnt; ++k) {
+            const float dist = sqrtf(l_d2[k * kBlock]);
             rho += c.m * poly6(c, dist);
+            if (total < kMaxNeighbours) nlist[(size_t)total * c.n + t] = l_slot[k * kBlock];
+            ++total;
         }
-    }
+        cnt = 0;
+    };
+    auto visit = [&](uint32_t start, uint32_t end) {
+        for (uint32_t u = start; u < end; u += kChunk) {
+            if (cnt > kListCap - kChunk) drain();
+            const uint32_t last = end - 1;
+            float4 q[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) q[j] = posrho[min(u + (uint32_t)j, last)];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+                const uint32_t v = u + (uint32_t)j;
+                const float rx = p.x - q[j].x, ry = p.y - q[j].y, rz = p.z - q[j].z;
+                const float d2 = (rx * rx + ry * ry) + rz * rz;
+                // dist < h  <=>  !(d2 >= cut2), NaN included (sqrtf(NaN) >= h is false)
+                const bool ok = (v < end) & (v != t) & !(d2 >= c.cut2);
+                l_d2[cnt * kBlock] = d2;
+                l_slot[cnt * kBlock] = v;
+                cnt += ok ? 1 : 0;
+            }
+        }
+    };
+    for_each_neighbour_cell(skey[t], range, c, visit, drain);
+
     const float wall = c.wall_density ? wall_density(c, p.x, p.y, p.z) : 0.0f;
     reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = rho + wall;
     reinterpret_cast<float*>(velp)[4 * (size_t)t + 3] = eos(c, rho);
+    ncount[t] = total;
 }
 
 // ------------------------------------------------------------------------------------------
 // 6. forces + symplectic Euler + cube boundary (update.comp:134-232): sorted buffer -> buffer 1.
+//    Neighbours come from the density pass's list; a particle with more than kMaxNeighbours
+//    neighbours walks its 27 cells again instead (same set, same order).
 // ------------------------------------------------------------------------------------------
 struct V3 {
     float x, y, z;
@@ -381,53 +447,64 @@ __device__ __forceinline__ void integrate_axis(const StepConsts& c, float pf, fl
     }
 }
 
+// Pressure and viscosity contributions of one accepted neighbour (update.comp:178-187).
+struct ForceSums {
+    float pfx, pfy, pfz, vfx, vfy, vfz;
+
+    __device__ __forceinline__ void add(const StepConsts& c, const float4& p, const float4& pv, const float4& q,
+                                        const float4& qv) {
+        const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+        const float dist = sqrtf((rx * rx + ry * ry) + rz * rz);
+        const float pressure = (pv.w + qv.w) / (2.0f * q.w);
+        if (pressure > 0.0f) {
+            const V3 sw = spiky(c, rx, ry, rz, dist + 1e-16f);
+            const float mp = c.m * pressure;
+            pfx = pfx - mp * sw.x;
+            pfy = pfy - mp * sw.y;
+            pfz = pfz - mp * sw.z;
+        }
+        const float wv = (c.h - dist) * c.visc;
+        vfx = vfx + (c.m * ((qv.x - pv.x) / q.w)) * wv;
+        vfy = vfy + (c.m * ((qv.y - pv.y) / q.w)) * wv;
+        vfz = vfz + (c.m * ((qv.z - pv.z) / q.w)) * wv;
+    }
+};
+
 __global__ __launch_bounds__(kBlock) void k_update(const float4* __restrict__ posrho_in,
                                                    const float4* __restrict__ velp_in,
                                                    const uint32_t* __restrict__ skey,
-                                                   const uint2* __restrict__ range, float4* __restrict__ posrho_out,
+                                                   const uint2* __restrict__ range,
+                                                   const uint32_t* __restrict__ nlist,
+                                                   const uint32_t* __restrict__ ncount, float4* __restrict__ posrho_out,
                                                    float4* __restrict__ velp_out, StepConsts c, float dt) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= c.n) return;
     const float4 p = posrho_in[t];
     const float4 pv = velp_in[t];
-    const uint32_t key = skey[t];
-    const int R = c.R;
-    const int cx = (int)(key % (uint32_t)R);
-    const int cy = (int)((key / (uint32_t)R) % (uint32_t)R);
-    const int cz = (int)(key / ((uint32_t)R * (uint32_t)R));
+    const uint32_t nn = ncount[t];
 
-    float pfx = 0.0f, pfy = 0.0f, pfz = 0.0f;
-    float vfx = 0.0f, vfy = 0.0f, vfz = 0.0f;
-#pragma unroll 1
-    for (int b = 0; b < 27; ++b) {
-        const int nx = cx + (b / 9 - 1);
-        const int ny = cy + ((b / 3) % 3 - 1);
-        const int nz = cz + (b % 3 - 1);
-        if (nx < 0 || ny < 0 || nz < 0 || nx >= R || ny >= R || nz >= R) continue;
-        const uint2 rg = range[(nz * R + ny) * R + nx];
-        const uint32_t end = min(rg.x + rg.y, c.n);
-        for (uint32_t u = rg.x; u < end; ++u) {
-            if (u == t) continue;
-            const float4 q = posrho_in[u];
-            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-            const float d2 = (rx * rx + ry * ry) + rz * rz;
-            if (d2 >= c.cut2) continue;
-            const float dist = sqrtf(d2);
-            const float4 qv = velp_in[u];
-            const float pressure = (pv.w + qv.w) / (2.0f * q.w);
-            if (pressure > 0.0f) {
-                const V3 w = spiky(c, rx, ry, rz, dist + 1e-16f);
-                const float mp = c.m * pressure;
-                pfx = pfx - mp * w.x;
-                pfy = pfy - mp * w.y;
-                pfz = pfz - mp * w.z;
-            }
-            const float wv = (c.h - dist) * c.visc;
-            vfx = vfx + (c.m * ((qv.x - pv.x) / q.w)) * wv;
-            vfy = vfy + (c.m * ((qv.y - pv.y) / q.w)) * wv;
-            vfz = vfz + (c.m * ((qv.z - pv.z) / q.w)) * wv;
+    ForceSums f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (nn <= kMaxNeighbours) {
+        uint32_t u_next = nn ? nlist[t] : 0u;
+        for (uint32_t k = 0; k < nn; ++k) {
+            const uint32_t u = u_next;
+            if (k + 1 < nn) u_next = nlist[(size_t)(k + 1) * c.n + t];
+            f.add(c, p, pv, posrho_in[u], velp_in[u]);
         }
+    } else {
+        auto visit = [&](uint32_t start, uint32_t end) {
+            for (uint32_t u = start; u < end; ++u) {
+                if (u == t) continue;
+                const float4 q = posrho_in[u];
+                const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                if ((rx * rx + ry * ry) + rz * rz >= c.cut2) continue;
+                f.add(c, p, pv, q, velp_in[u]);
+            }
+        };
+        auto nothing = []() {};
+        for_each_neighbour_cell(skey[t], range, c, visit, nothing);
     }
+
     V3 ext{c.g[0] * p.w, c.g[1] * p.w, c.g[2] * p.w};
     V3 mouse{0.0f, 0.0f, 0.0f}, wall{0.0f, 0.0f, 0.0f};
     if (c.mouse_force) mouse = mouse_force(c, p.x, p.y, p.z, pv.w);
@@ -438,9 +515,9 @@ __global__ __launch_bounds__(kBlock) void k_update(const float4* __restrict__ po
 
     const float denom = p.w + 1e-16f;
     float4 po, vo;
-    integrate_axis(c, pfx, vfx, ext.x, denom, pv.x, p.x, dt, vo.x, po.x);
-    integrate_axis(c, pfy, vfy, ext.y, denom, pv.y, p.y, dt, vo.y, po.y);
-    integrate_axis(c, pfz, vfz, ext.z, denom, pv.z, p.z, dt, vo.z, po.z);
+    integrate_axis(c, f.pfx, f.vfx, ext.x, denom, pv.x, p.x, dt, vo.x, po.x);
+    integrate_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, vo.y, po.y);
+    integrate_axis(c, f.pfz, f.vfz, ext.z, denom, pv.z, p.z, dt, vo.z, po.z);
     po.w = p.w;
     vo.w = pv.w;
     posrho_out[t] = po;
@@ -509,14 +586,14 @@ hipError_t launch_gather(const DeviceBuffers& b, const StepConsts& c, hipStream_
 hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
     if (c.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_density, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.posrho[1], b.velp[1],
-                       b.skey, b.range, c);
+                       b.skey, b.range, b.nlist, b.ncount, c);
     return hipGetLastError();
 }
 
 hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, hipStream_t s) {
     if (c.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_update, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.posrho[1], b.velp[1],
-                       b.skey, b.range, b.posrho[0], b.velp[0], c, dt);
+                       b.skey, b.range, b.nlist, b.ncount, b.posrho[0], b.velp[0], c, dt);
     return hipGetLastError();
 }
 

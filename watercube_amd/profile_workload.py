@@ -1,0 +1,53 @@
+"""Fixed workload for profiler runs (rocprofv3 kernel traces and PMC counters).
+
+    python3 -m watercube_amd.profile_workload [--workload block_1m] [--steps 20] [--mode graph|eager]
+
+Uploads the workload's initial state (the upload's record-split kernel `k_split` reads 32 bytes
+and writes 32 bytes per particle: a plain copy of known size, used to calibrate the byte
+counters), runs a few warm-up steps, then `--steps` steps either as the production captured
+graph or as eager per-pass launches, and prints one line per stage.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="block_1m")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
+    args = ap.parse_args()
+
+    from watercube_amd import Fluid, scenes
+
+    p, parts, dt, _ = scenes.workload(args.workload)
+    print(f"profile_workload: {args.workload} n={p.num_particles} copy-calibration bytes/dispatch "
+          f"read={32 * p.num_particles} write={32 * p.num_particles}", flush=True)
+    with Fluid(p, parts) as f:
+        f.step(dt, args.warmup)
+        f.synchronize()
+        t0 = time.perf_counter()
+        if args.mode == "graph":
+            f.step(dt, args.steps)
+        else:
+            for _ in range(args.steps):
+                f.sort_.run()
+                f.runDensityProg()
+                f.runUpdateProg(dt)
+        f.synchronize()
+        dt_wall = time.perf_counter() - t0
+    print(f"profile_workload: {args.steps} {args.mode} steps in {dt_wall * 1e3:.2f} ms", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
