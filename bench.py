@@ -102,6 +102,11 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     with Fluid(p, parts) as f:
+        tuning = f.tune(dt, reps=5)  # untimed: picks the fastest variant of each neighbour pass
+        print(f"# tuned variants: density {tuning['variants'][tuning['density_variant']]} "
+              f"update {tuning['variants'][tuning['update_variant']]}", file=sys.stderr, flush=True)
+        print("# variant ms density: " + " ".join(f"{x:.4f}" for x in tuning["density_ms"]) +
+              " | update: " + " ".join(f"{x:.4f}" for x in tuning["update_ms"]), file=sys.stderr, flush=True)
         f.step(dt, args.warmup)
         f.synchronize()
         t0 = time.perf_counter()
@@ -140,7 +145,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "particles": n, "grid_res": int(p.grid_res), "dt": dt,
-                   "method": "plain", "kernel_radius": float(p.kernel_radius)},
+                   "method": "plain", "kernel_radius": float(p.kernel_radius),
+                   "tuning": {"density": tuning["variants"][tuning["density_variant"]],
+                              "update": tuning["variants"][tuning["update_variant"]]}},
         "simulated_s_per_s": dt * args.steps / elapsed,
         "pass_ms": per_launch_ms,
         "roofline": {

@@ -205,6 +205,27 @@ WC_API int wc_device_view_get(wc_ctx* ctx, wc_device_view* out);
 WC_API int wc_profile_steps(wc_ctx* ctx, float dt, uint32_t nsteps, double ms_total[WC_NUM_PASSES]);
 WC_API const char* wc_pass_name(int k);
 
+/* Tuning of the two neighbour passes (density, force).  Variant v in [0, wc_num_variants()) is
+ * a (threads per workgroup, LDS list entries per lane, neighbour slots in flight per lane, drain
+ * the list after every dx plane or only when full) choice; every variant computes bit-identical
+ * results, only speed differs.  No reference
+ * counterpart: the reference dispatches fixed 128-invocation work groups (src/core/util.h:18). */
+typedef struct wc_tuning {
+    int32_t density_variant;
+    int32_t update_variant;
+} wc_tuning;
+WC_API int wc_num_variants(void);
+WC_API int wc_variant_info(int v, int32_t* block, int32_t* list_cap, int32_t* chunk, int32_t* plane_drain);
+WC_API int wc_get_tuning(const wc_ctx* ctx, wc_tuning* out);
+WC_API int wc_set_tuning(wc_ctx* ctx, const wc_tuning* tuning);
+
+/* Autotuner: sorts the current state once, then times every variant of the density pass and of
+ * the force pass (reps launches each, HIP events on the context's stream) and keeps the fastest
+ * of each for later steps.  Particle buffer 1 is restored afterwards; buffer 2 (the sorted copy
+ * the next step rebuilds anyway) is left sorted.  ms_density / ms_update, when not NULL, receive
+ * wc_num_variants() average launch times in milliseconds. */
+WC_API int wc_tune(wc_ctx* ctx, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update);
+
 /* Text of the last failure on the calling thread ("" if none). */
 WC_API const char* wc_last_error(void);
 

@@ -108,3 +108,154 @@ def test_graph_step_equals_eager_passes(oracle):
             b.runDensityProg()
             b.runUpdateProg(dt)
         _check_records(a.particles(1), b.particles(1), "graph vs eager")
+
+
+def test_every_tuning_variant_is_bit_identical(oracle):
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 5)
+    nv = L.lib().wc_num_variants()
+    for v in range(nv):
+        _log(f"variant {v}: {L.variants()[v]}")
+        with Fluid(p, parts) as f:
+            f.set_tuning(v, v)
+            f.step(dt, 5)
+            _check_records(f.particles(1), st.buf1, f"variant {v} {L.variants()[v]}")
+
+
+def test_autotuner_keeps_state_and_results(oracle):
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 4)
+    with Fluid(p, parts) as f:
+        f.step(dt, 2)
+        before = f.particles(1)
+        info = f.tune(dt, reps=2)
+        _log(f"tuned: {info['density_variant']} {info['update_variant']}")
+        assert f.particles(1).tobytes() == before.tobytes()
+        assert all(x > 0 for x in info["density_ms"]) and all(x > 0 for x in info["update_ms"])
+        f.step(dt, 2)
+        _check_records(f.particles(1), st.buf1, "after tuning")
+
+
+def _params_for(n, preset=L.WC_PRESET_DAM_BREAK_16K):
+    p = L.preset(preset)
+    p.num_particles = n
+    return p
+
+
+def _records(pos, vel=None):
+    a = np.zeros(len(pos), L.PARTICLE_DTYPE)
+    a["position"] = pos
+    if vel is not None:
+        a["velocity"] = vel
+    return a
+
+
+def _trajectory_case(p, parts, dt, steps, oracle, what):
+    _log(f"case {what}: n={p.num_particles} steps={steps}")
+    st = oracle.OracleState(p, parts)
+    st.step(dt, steps)
+    with Fluid(p, parts) as f:
+        f.step(dt, steps)
+        f.synchronize()
+        _check_records(f.particles(1), st.buf1, what)
+        if p.num_particles:
+            assert np.array_equal(f.sort_.getCountBuffer(), st.counts)
+            assert np.array_equal(f.sort_.getOffsetBuffer(), st.offsets)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 65, 257])
+def test_tiny_scenes(n, oracle):
+    rng = np.random.default_rng(n)
+    parts = _records(rng.uniform(0.3, 0.36, (n, 3)).astype(np.float32))
+    _trajectory_case(_params_for(n), parts, 2e-4, 10, oracle, f"tiny n={n}")
+
+
+def test_particles_outside_the_cube(oracle):
+    """Positions below 0, above size, and exactly on cell boundaries: the cell hash clamps and the
+    boundary pass pulls them back, identically on both sides."""
+    p = _params_for(0)
+    b = float(p.bin_size)
+    pos = [[-0.2, 0.5, 0.5], [1.3, 0.5, 0.5], [0.5, -1e-3, 2.0], [b, 2 * b, 3 * b], [b, 2 * b, 3 * b + 1e-7],
+           [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [0.5, 0.5, 0.5], [0.5 + 0.01, 0.5, 0.5]]
+    p.num_particles = len(pos)
+    _trajectory_case(p, _records(np.float32(pos), np.float32([[3, -4, 5]] * len(pos))), 2e-4, 5, oracle, "outside")
+
+
+def test_dense_cluster_uses_fallback_path(oracle):
+    """400 particles packed into a quarter-cell: every particle has ~399 neighbours, more than the
+    neighbour list holds, so the force pass walks the cells again; results must not change."""
+    p = _params_for(400)
+    h = float(p.kernel_radius)
+    rng = np.random.default_rng(9)
+    parts = _records(rng.uniform(0.5, 0.5 + h / 4, (400, 3)).astype(np.float32))
+    _trajectory_case(p, parts, 2e-5, 3, oracle, "dense cluster")
+
+
+def test_mouse_force_trajectory(oracle):
+    p, parts, dt, _ = scenes.workload("reference_default")
+    p.mouse_force = 1
+    p.camera_position[:] = (0.3, 0.2, -2.0)
+    p.mouse_ray_direction[:] = (0.0, 0.05, 0.998749)
+    _trajectory_case(p, parts, dt, 20, oracle, "mouse force")
+
+
+def test_long_trajectory_dam_break(oracle):
+    """1000 steps of the 16,384-particle dam break: bit-identical, so the relative error bound of
+    1e-3 after 1000 steps holds with error 0."""
+    p, parts, dt, steps = scenes.workload("dam_break_16k")
+    _trajectory_case(p, parts, dt, steps, oracle, "dam break 1000 steps")
+
+
+def test_block_1m_properties_after_run(oracle):
+    """Full-size invariants after 200 steps of the 1,000,000-particle block: counts sum to N,
+    offsets are their exclusive scan, the sorted order is a permutation whose cells ascend, every
+    slot range holds only its cell, all values finite and inside the cube."""
+    p, parts, dt, _ = scenes.workload("block_1m")
+    with Fluid(p, parts) as f:
+        f.step(dt, 200)
+        f.sort_.run()
+        f.synchronize()
+        counts = f.sort_.getCountBuffer()
+        offsets = f.sort_.getOffsetBuffer()
+        order = f.sort_.getSortedBuffer()
+        keys_in = f.sort_.getCellKeys()
+        buf1 = f.particles(1)
+        buf2 = f.particles(2)
+    n = p.num_particles
+    assert counts.sum() == n
+    assert np.array_equal(offsets, np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.uint32))
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
+    k_sorted = keys_in[order]
+    assert np.all(np.diff(k_sorted.astype(np.int64)) >= 0)
+    assert np.array_equal(np.repeat(np.arange(len(counts)), counts), k_sorted)
+    # stable: inside a cell, input order ascends
+    same = np.diff(k_sorted.astype(np.int64)) == 0
+    assert np.all(np.diff(order.astype(np.int64))[same] > 0)
+    assert buf2.tobytes() == buf1[order].tobytes()
+    pos = buf1["position"]
+    assert np.isfinite(pos).all() and (pos >= p.border).all() and (pos <= 1 - p.border).all()
+    # and one more step from this state matches the CPU restatement bit for bit
+    st = oracle.OracleState(p, buf1)
+    st.step(dt, 1)
+    with Fluid(p, buf1) as g:
+        g.step(dt, 1)
+        _check_records(g.particles(1), st.buf1, "block_1m step from step-200 state")
+
+
+def test_cpp_harness_matches_oracle(tmp_path, oracle):
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(L.__file__).resolve().parent / "wc_harness"
+    out = tmp_path / "state.bin"
+    r = subprocess.run([str(exe), "dam_break_16k", "100", "--out", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    _log(r.stdout.strip())
+    got = np.fromfile(out, L.PARTICLE_DTYPE)
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 100)
+    _check_records(got, st.buf1, "C++ harness 100 steps")

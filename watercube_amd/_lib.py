@@ -97,6 +97,10 @@ class wc_device_view(C.Structure):
     ]
 
 
+class wc_tuning(C.Structure):
+    _fields_ = [("density_variant", C.c_int32), ("update_variant", C.c_int32)]
+
+
 _P = C.POINTER
 _u32p = _P(C.c_uint32)
 _f32p = _P(C.c_float)
@@ -127,6 +131,11 @@ SIGNATURES = {
     "wc_device_view_get": (C.c_int, [_vp, _P(wc_device_view)]),
     "wc_profile_steps": (C.c_int, [_vp, C.c_float, C.c_uint32, _P(C.c_double)]),
     "wc_pass_name": (C.c_char_p, [C.c_int]),
+    "wc_num_variants": (C.c_int, []),
+    "wc_variant_info": (C.c_int, [C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),
+    "wc_get_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
+    "wc_set_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
+    "wc_tune": (C.c_int, [_vp, C.c_float, C.c_uint32, _P(wc_tuning), _P(C.c_double), _P(C.c_double)]),
     "wc_last_error": (C.c_char_p, []),
     "wc_abi_version": (C.c_int, []),
     "wc_target_arch": (C.c_char_p, []),
@@ -171,6 +180,17 @@ def f3(v) -> C.Array:
 def u32ptr(a: np.ndarray):
     assert a.dtype == np.uint32 and a.flags.c_contiguous
     return a.ctypes.data_as(_u32p)
+
+
+def variants() -> list[tuple[int, int, int, int]]:
+    """(threads per workgroup, LDS list entries per lane, slots in flight, per-plane drain) of
+    each variant of the neighbour passes."""
+    out = []
+    for v in range(lib().wc_num_variants()):
+        b, c, k, d = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().wc_variant_info(v, C.byref(b), C.byref(c), C.byref(k), C.byref(d)), "wc_variant_info")
+        out.append((b.value, c.value, k.value, d.value))
+    return out
 
 
 def preset(which: int) -> wc_params:

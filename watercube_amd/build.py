@@ -16,6 +16,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libwatercube.so"
+HARNESS = PKG / "wc_harness"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["wc_api.hip", "wc_kernels.hip", "wc_scenes.cpp"]
@@ -48,6 +49,15 @@ def build(verbose: bool = False) -> Path:
         sys.stderr.write(proc.stdout + proc.stderr)
         raise RuntimeError(f"hipcc failed ({proc.returncode}) building {LIB.name}")
     os.replace(tmp, LIB)
+    # headless C++ driver over the C ABI (host code only)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", str(HARNESS), str(CSRC / "wc_harness.cpp"),
+           "-L", str(PKG), "-lwatercube", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError("g++ failed building wc_harness")
     return LIB
 
 

@@ -106,6 +106,8 @@ struct wc_ctx {
     hipGraphExec_t graph_exec = nullptr;
     float graph_dt = 0.0f;
     bool graph_valid = false;
+    int density_variant = 2;  // (64 threads, 32 list entries, 4 in flight, per-plane drain) until tuned
+    int update_variant = 2;
 };
 
 namespace {
@@ -124,7 +126,6 @@ int free_ctx_buffers(wc_ctx* c) {
     DeviceBuffers& b = c->b;
     void* ptrs[] = {b.posrho[0], b.posrho[1], b.velp[0], b.velp[1], b.key,         b.rank,
                     b.slot_tmp,  b.sorted,    b.skey,    b.counts,  b.range,       b.scan_status,
-                    b.nlist,     b.ncount,
                     b.scan_ticket, b.error_flag, c->staging};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -152,8 +153,8 @@ int enqueue_sort(wc_ctx* c) {
 int enqueue_step(wc_ctx* c, float dt) {
     int rc = enqueue_sort(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_density(c->b, c->consts, c->stream));
-    WC_HIP(wc::launch_update(c->b, c->consts, dt, c->stream));
+    WC_HIP(wc::launch_density(c->b, c->consts, c->density_variant, c->stream));
+    WC_HIP(wc::launch_update(c->b, c->consts, dt, c->update_variant, c->stream));
     return WC_OK;
 }
 
@@ -281,8 +282,6 @@ WC_API int wc_create(const wc_params* p, wc_ctx** out) {
     alloc((void**)&b.skey, n * 4);
     alloc((void**)&b.counts, cells * 4 + 64);
     alloc((void**)&b.range, cells * 8 + 64);
-    alloc((void**)&b.nlist, n * 4 * wc::kMaxNeighbours);
-    alloc((void**)&b.ncount, n * 4);
     alloc((void**)&b.scan_status, (size_t)tiles * 8 + 8);
     alloc((void**)&b.scan_ticket, 64);
     alloc((void**)&b.error_flag, 64);
@@ -300,7 +299,6 @@ WC_API int wc_create(const wc_params* p, wc_ctx** out) {
     if (e == hipSuccess) e = hipMemsetAsync(b.sorted, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.skey, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.key, 0, n * 4, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.ncount, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.posrho[0], 0, n * sizeof(float4), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.posrho[1], 0, n * sizeof(float4), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b.velp[0], 0, n * sizeof(float4), c->stream);
@@ -387,7 +385,7 @@ WC_API int wc_density(wc_ctx* c) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     int rc = set_device(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_density(c->b, c->consts, c->stream));
+    WC_HIP(wc::launch_density(c->b, c->consts, c->density_variant, c->stream));
     return WC_OK;
 }
 
@@ -395,7 +393,7 @@ WC_API int wc_update(wc_ctx* c, float dt) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     int rc = set_device(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_update(c->b, c->consts, dt, c->stream));
+    WC_HIP(wc::launch_update(c->b, c->consts, dt, c->update_variant, c->stream));
     return WC_OK;
 }
 
@@ -517,9 +515,9 @@ WC_API int wc_profile_steps(wc_ctx* c, float dt, uint32_t nsteps, double ms_tota
         if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
         if (err == hipSuccess) err = wc::launch_gather(c->b, c->consts, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[4], c->stream);
-        if (err == hipSuccess) err = wc::launch_density(c->b, c->consts, c->stream);
+        if (err == hipSuccess) err = wc::launch_density(c->b, c->consts, c->density_variant, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[5], c->stream);
-        if (err == hipSuccess) err = wc::launch_update(c->b, c->consts, dt, c->stream);
+        if (err == hipSuccess) err = wc::launch_update(c->b, c->consts, dt, c->update_variant, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[6], c->stream);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
@@ -534,6 +532,105 @@ WC_API int wc_profile_steps(wc_ctx* c, float dt, uint32_t nsteps, double ms_tota
     }
     for (auto& x : ev) (void)hipEventDestroy(x);
     if (err != hipSuccess) return fail(WC_ERR_HIP, "profiled step failed: %s", hipGetErrorString(err));
+    return WC_OK;
+}
+
+WC_API int wc_num_variants(void) { return wc::num_variants(); }
+
+WC_API int wc_variant_info(int v, int32_t* block, int32_t* list_cap, int32_t* chunk, int32_t* plane_drain) {
+    if (v < 0 || v >= wc::num_variants()) return fail(WC_ERR_INVALID_ARG, "variant %d outside [0, %d)", v, wc::num_variants());
+    const wc::PassVariant pv = wc::variant(v);
+    if (block) *block = pv.block;
+    if (list_cap) *list_cap = pv.list_cap;
+    if (chunk) *chunk = pv.chunk;
+    if (plane_drain) *plane_drain = pv.plane_drain ? 1 : 0;
+    return WC_OK;
+}
+
+WC_API int wc_get_tuning(const wc_ctx* c, wc_tuning* out) {
+    if (!c || !out) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    out->density_variant = c->density_variant;
+    out->update_variant = c->update_variant;
+    return WC_OK;
+}
+
+WC_API int wc_set_tuning(wc_ctx* c, const wc_tuning* t) {
+    if (!c || !t) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    const int nv = wc::num_variants();
+    if (t->density_variant < 0 || t->density_variant >= nv || t->update_variant < 0 || t->update_variant >= nv)
+        return fail(WC_ERR_INVALID_ARG, "variant outside [0, %d)", nv);
+    c->density_variant = t->density_variant;
+    c->update_variant = t->update_variant;
+    drop_graph(c);
+    return WC_OK;
+}
+
+WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
+    if (reps == 0) reps = 1;
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int nv = wc::num_variants();
+    const size_t n = c->params.num_particles;
+    std::vector<double> td(nv, 0.0), tu(nv, 0.0);
+    if (n > 0) {
+        // Keep buffer 1 (the state the caller sees); the passes below rewrite it.
+        float4 *save_pos = nullptr, *save_vel = nullptr;
+        WC_HIP(hipMalloc(&save_pos, n * sizeof(float4)));
+        if (hipMalloc(&save_vel, n * sizeof(float4)) != hipSuccess) {
+            (void)hipFree(save_pos);
+            return fail(WC_ERR_NO_MEMORY, "tuning copy of %zu particles", n);
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t err = hipEventCreate(&e0);
+        if (err == hipSuccess) err = hipEventCreate(&e1);
+        if (err == hipSuccess) err = hipMemcpyAsync(save_pos, c->b.posrho[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess) err = hipMemcpyAsync(save_vel, c->b.velp[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess) err = (enqueue_sort(c) == WC_OK) ? hipSuccess : hipErrorUnknown;
+        // Both passes are idempotent on the sorted buffer: density rewrites the same densities,
+        // the force pass rewrites buffer 1 from buffer 2 alone.
+        for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
+            for (int v = 0; v < nv && err == hipSuccess; ++v) {
+                auto launch = [&]() {
+                    return pass == 0 ? wc::launch_density(c->b, c->consts, v, c->stream)
+                                     : wc::launch_update(c->b, c->consts, dt, v, c->stream);
+                };
+                err = launch();  // warm-up
+                if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
+                for (uint32_t r = 0; r < reps && err == hipSuccess; ++r) err = launch();
+                if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+                if (err == hipSuccess) err = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                (pass == 0 ? td : tu)[v] = (double)ms / reps;
+            }
+            if (pass == 0 && err == hipSuccess) err = wc::launch_density(c->b, c->consts, c->density_variant, c->stream);
+        }
+        if (err == hipSuccess) err = hipMemcpyAsync(c->b.posrho[0], save_pos, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess) err = hipMemcpyAsync(c->b.velp[0], save_vel, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(save_pos);
+        (void)hipFree(save_vel);
+        if (err != hipSuccess) return fail(WC_ERR_HIP, "tuning failed: %s", hipGetErrorString(err));
+        int bd = 0, bu = 0;
+        for (int v = 1; v < nv; ++v) {
+            if (td[v] < td[bd]) bd = v;
+            if (tu[v] < tu[bu]) bu = v;
+        }
+        c->density_variant = bd;
+        c->update_variant = bu;
+        drop_graph(c);
+    }
+    if (chosen) {
+        chosen->density_variant = c->density_variant;
+        chosen->update_variant = c->update_variant;
+    }
+    for (int v = 0; v < nv; ++v) {
+        if (ms_density) ms_density[v] = td[v];
+        if (ms_update) ms_update[v] = tu[v];
+    }
     return WC_OK;
 }
 

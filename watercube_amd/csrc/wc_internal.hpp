@@ -14,7 +14,6 @@ constexpr int kScanItems = 16;        // counts per scan thread (four 16-byte lo
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 constexpr uint32_t kLookbackSpinLimit = 1u << 24;  // bounded wait in the scan look-back
-constexpr uint32_t kMaxNeighbours = 64;             // neighbour-list slots per particle
 
 // Everything one step's kernels read besides buffers, precomputed on the host in float so the
 // device sees exactly the bits the CPU transcription uses.
@@ -54,8 +53,6 @@ struct DeviceBuffers {
     uint32_t* skey;      // slot -> cell index of the sorted particle
     uint32_t* counts;    // R^3 atomic histogram; the scan reads and zeroes it
     uint2* range;        // R^3 (exclusive offset, count)
-    uint32_t* nlist;     // kMaxNeighbours x n: slot of the k-th accepted neighbour of slot t at k*n + t
-    uint32_t* ncount;    // n: accepted neighbours per slot (may exceed kMaxNeighbours)
     unsigned long long* scan_status;  // one look-back word per scan tile
     uint32_t* scan_ticket;            // dynamic tile counter
     uint32_t* error_flag;             // nonzero after a failed bounded wait
@@ -67,8 +64,17 @@ hipError_t launch_hash(const DeviceBuffers& b, const StepConsts& c, hipStream_t 
 hipError_t launch_scan(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
 hipError_t launch_place(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
 hipError_t launch_gather(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
-hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
-hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, hipStream_t stream);
+// Tuning variant of the neighbour passes; all variants produce identical results.
+struct PassVariant {
+    int block;     // threads per workgroup
+    int list_cap;  // LDS list entries per lane
+    int chunk;     // neighbour slots loaded per step of a lane's walk
+    bool plane_drain;  // drain the list after every dx plane (else only when full and at the end)
+};
+int num_variants();
+PassVariant variant(int v);
+hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, int variant, hipStream_t stream);
+hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, int variant, hipStream_t stream);
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n,
                                 hipStream_t stream);
 hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,

@@ -6,6 +6,8 @@
 // (-ffp-contract=off), every expression associated left to right as the GLSL source writes it.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "wc_internal.hpp"
 
 namespace wc {
@@ -278,21 +280,61 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------------------------------
-// 5. density + pressure (density.comp:81-137), in place on the sorted buffer.
+// Neighbour search shared by the density and the force pass.
 //
-//    Neighbour search: each lane walks its own slot range of the current neighbour cell four
-//    slots at a time (four loads in flight).  Candidate tests are branch-free: every candidate
-//    is written to the lane's next LDS list entry and the entry count advances by the outcome.
-//    The list is drained after every dx plane (or when it could overflow): the density terms are
-//    then summed over compacted entries, with all lanes busy, in enumeration order.  The drain
-//    also appends each accepted neighbour's slot to the particle's neighbour list in HBM
-//    (nlist[k * n + t], k < kMaxNeighbours), which the force pass reads instead of searching
-//    again: positions do not change between the two passes, so the accepted set and its order
-//    are the same.
+// Each lane owns one particle.  The 27 neighbour cells are visited in NEIGHBORHOOD order with a
+// wave-uniform cell loop (for_each_neighbour_cell); inside a cell each lane walks its own slot
+// range CHUNK slots at a time (CHUNK loads in flight).  Candidate tests are branch-free: every
+// candidate is written to the lane's next entry of an LDS list and the entry count advances by
+// the test's outcome.  The list is drained after every dx plane, or when the next chunk could
+// overflow it, or only when full and at the end (PLANE_DRAIN false): the per-neighbour arithmetic then runs over compacted entries with all lanes
+// busy instead of under a mask where ~15% of the lanes accept a given candidate.  Entries are
+// drained in enumeration order, so every sum is formed exactly as the reference forms it, and
+// the result does not depend on BLOCK, CAP, CHUNK or PLANE_DRAIN (the tuning knobs of the passes).
 // ------------------------------------------------------------------------------------------
-constexpr int kChunk = 4;
-constexpr int kListCap = 16;
+template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, typename Entry, typename Drain>
+__device__ __forceinline__ void search_neighbours(uint32_t key, uint32_t t, float px, float py, float pz,
+                                                  const float4* __restrict__ pos, const uint2* __restrict__ range,
+                                                  const StepConsts& c, Entry* __restrict__ list, Drain& drain_fn) {
+    static_assert(CAP >= CHUNK, "list must hold a chunk");
+    int cnt = 0;
+    auto drain = [&]() {
+        drain_fn(cnt);
+        cnt = 0;
+    };
+    auto visit = [&](uint32_t start, uint32_t end) {
+        for (uint32_t u = start; u < end; u += CHUNK) {
+            if (cnt > CAP - CHUNK) drain();
+            const uint32_t last = end - 1;
+            float4 q[CHUNK];
+#pragma unroll
+            for (int j = 0; j < CHUNK; ++j) q[j] = pos[min(u + (uint32_t)j, last)];
+#pragma unroll
+            for (int j = 0; j < CHUNK; ++j) {
+                const uint32_t v = u + (uint32_t)j;
+                const float rx = px - q[j].x, ry = py - q[j].y, rz = pz - q[j].z;
+                const float d2 = (rx * rx + ry * ry) + rz * rz;
+                // dist < h  <=>  !(d2 >= cut2), NaN included (sqrtf(NaN) >= h is false)
+                const bool ok = (v < end) & (v != t) & !(d2 >= c.cut2);
+                if constexpr (std::is_same<Entry, float>::value)
+                    list[cnt * BLOCK] = d2;  // density: squared distance
+                else
+                    list[cnt * BLOCK] = v;   // force pass: the neighbour's slot
+                cnt += ok ? 1 : 0;
+            }
+        }
+    };
+    auto end_plane = [&]() {
+        if (PLANE_DRAIN) drain();
+    };
+    for_each_neighbour_cell(key, range, c, visit, end_plane);
+    drain();
+}
 
+// ------------------------------------------------------------------------------------------
+// 5. density + pressure (density.comp:81-137), in place on the sorted buffer.  The LDS list
+//    holds each accepted neighbour's squared distance.
+// ------------------------------------------------------------------------------------------
 __device__ __forceinline__ float wall_density(const StepConsts& c, float x, float y, float z) {
     // density.comp:57-79; the z branch's else-test reads p.y (line 74)
     float d = 0.0f;
@@ -311,62 +353,27 @@ __device__ __forceinline__ float eos(const StepConsts& c, float rho) {
     return c.p0 + c.k * (((x * x) * x) - 1.0f);
 }
 
-__global__ __launch_bounds__(kBlock) void k_density(float4* __restrict__ posrho, float4* __restrict__ velp,
-                                                    const uint32_t* __restrict__ skey,
-                                                    const uint2* __restrict__ range, uint32_t* __restrict__ nlist,
-                                                    uint32_t* __restrict__ ncount, StepConsts c) {
-    __shared__ float s_d2[kListCap * kBlock];
-    __shared__ uint32_t s_slot[kListCap * kBlock];
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN>
+__global__ __launch_bounds__(BLOCK) void k_density(float4* __restrict__ posrho, float4* __restrict__ velp,
+                                                   const uint32_t* __restrict__ skey,
+                                                   const uint2* __restrict__ range, StepConsts c) {
+    __shared__ float s_list[CAP * BLOCK];
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (t >= c.n) return;
     const float4 p = posrho[t];
-    float* l_d2 = s_d2 + threadIdx.x;
-    uint32_t* l_slot = s_slot + threadIdx.x;
-
+    float* list = s_list + threadIdx.x;
     float rho = c.m * poly6(c, 0.0f);
-    uint32_t total = 0;
-    int cnt = 0;
-    auto drain = [&]() {
-        for (int k = 0; k < cnt; ++k) {
-            const float dist = sqrtf(l_d2[k * kBlock]);
-            rho += c.m * poly6(c, dist);
-            if (total < kMaxNeighbours) nlist[(size_t)total * c.n + t] = l_slot[k * kBlock];
-            ++total;
-        }
-        cnt = 0;
+    auto drain = [&](int cnt) {
+        for (int k = 0; k < cnt; ++k) rho += c.m * poly6(c, sqrtf(list[k * BLOCK]));
     };
-    auto visit = [&](uint32_t start, uint32_t end) {
-        for (uint32_t u = start; u < end; u += kChunk) {
-            if (cnt > kListCap - kChunk) drain();
-            const uint32_t last = end - 1;
-            float4 q[kChunk];
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) q[j] = posrho[min(u + (uint32_t)j, last)];
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) {
-                const uint32_t v = u + (uint32_t)j;
-                const float rx = p.x - q[j].x, ry = p.y - q[j].y, rz = p.z - q[j].z;
-                const float d2 = (rx * rx + ry * ry) + rz * rz;
-                // dist < h  <=>  !(d2 >= cut2), NaN included (sqrtf(NaN) >= h is false)
-                const bool ok = (v < end) & (v != t) & !(d2 >= c.cut2);
-                l_d2[cnt * kBlock] = d2;
-                l_slot[cnt * kBlock] = v;
-                cnt += ok ? 1 : 0;
-            }
-        }
-    };
-    for_each_neighbour_cell(skey[t], range, c, visit, drain);
-
+    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, p.x, p.y, p.z, posrho, range, c, list, drain);
     const float wall = c.wall_density ? wall_density(c, p.x, p.y, p.z) : 0.0f;
     reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = rho + wall;
     reinterpret_cast<float*>(velp)[4 * (size_t)t + 3] = eos(c, rho);
-    ncount[t] = total;
 }
 
 // ------------------------------------------------------------------------------------------
 // 6. forces + symplectic Euler + cube boundary (update.comp:134-232): sorted buffer -> buffer 1.
-//    Neighbours come from the density pass's list; a particle with more than kMaxNeighbours
-//    neighbours walks its 27 cells again instead (same set, same order).
 // ------------------------------------------------------------------------------------------
 struct V3 {
     float x, y, z;
@@ -470,40 +477,26 @@ struct ForceSums {
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k_update(const float4* __restrict__ posrho_in,
-                                                   const float4* __restrict__ velp_in,
-                                                   const uint32_t* __restrict__ skey,
-                                                   const uint2* __restrict__ range,
-                                                   const uint32_t* __restrict__ nlist,
-                                                   const uint32_t* __restrict__ ncount, float4* __restrict__ posrho_out,
-                                                   float4* __restrict__ velp_out, StepConsts c, float dt) {
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN>
+__global__ __launch_bounds__(BLOCK) void k_update(const float4* __restrict__ posrho_in,
+                                                  const float4* __restrict__ velp_in,
+                                                  const uint32_t* __restrict__ skey,
+                                                  const uint2* __restrict__ range, float4* __restrict__ posrho_out,
+                                                  float4* __restrict__ velp_out, StepConsts c, float dt) {
+    __shared__ uint32_t s_list[CAP * BLOCK];
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (t >= c.n) return;
     const float4 p = posrho_in[t];
     const float4 pv = velp_in[t];
-    const uint32_t nn = ncount[t];
-
+    uint32_t* list = s_list + threadIdx.x;
     ForceSums f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (nn <= kMaxNeighbours) {
-        uint32_t u_next = nn ? nlist[t] : 0u;
-        for (uint32_t k = 0; k < nn; ++k) {
-            const uint32_t u = u_next;
-            if (k + 1 < nn) u_next = nlist[(size_t)(k + 1) * c.n + t];
+    auto drain = [&](int cnt) {
+        for (int k = 0; k < cnt; ++k) {
+            const uint32_t u = list[k * BLOCK];
             f.add(c, p, pv, posrho_in[u], velp_in[u]);
         }
-    } else {
-        auto visit = [&](uint32_t start, uint32_t end) {
-            for (uint32_t u = start; u < end; ++u) {
-                if (u == t) continue;
-                const float4 q = posrho_in[u];
-                const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-                if ((rx * rx + ry * ry) + rz * rz >= c.cut2) continue;
-                f.add(c, p, pv, q, velp_in[u]);
-            }
-        };
-        auto nothing = []() {};
-        for_each_neighbour_cell(skey[t], range, c, visit, nothing);
-    }
+    };
+    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, p.x, p.y, p.z, posrho_in, range, c, list, drain);
 
     V3 ext{c.g[0] * p.w, c.g[1] * p.w, c.g[2] * p.w};
     V3 mouse{0.0f, 0.0f, 0.0f}, wall{0.0f, 0.0f, 0.0f};
@@ -583,18 +576,55 @@ hipError_t launch_gather(const DeviceBuffers& b, const StepConsts& c, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
+// Tuning variants of the two neighbour passes: (block size, LDS list entries per lane, slots
+// in flight per lane, drain after every dx plane).  Every variant computes identical bits.
+#define WC_VARIANTS(X)     \
+    X(256, 32, 4, true)    \
+    X(128, 32, 4, true)    \
+    X(64, 32, 4, true)     \
+    X(64, 32, 4, false)    \
+    X(64, 40, 4, false)    \
+    X(64, 48, 4, false)    \
+    X(64, 64, 4, false)    \
+    X(128, 48, 4, false)   \
+    X(128, 32, 2, true)    \
+    X(256, 16, 4, true)
+
+static const PassVariant kVariants[] = {
+#define WC_ROW(B, C, K, P) {B, C, K, P},
+    WC_VARIANTS(WC_ROW)
+#undef WC_ROW
+};
+
+int num_variants() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+PassVariant variant(int v) { return kVariants[(v >= 0 && v < num_variants()) ? v : 0]; }
+
+hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, int v, hipStream_t s) {
     if (c.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_density, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.posrho[1], b.velp[1],
-                       b.skey, b.range, b.nlist, b.ncount, c);
-    return hipGetLastError();
+    const PassVariant pv = variant(v);
+#define WC_LAUNCH(B, C, K, P)                                                                                 \
+    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                         \
+        hipLaunchKernelGGL((k_density<B, C, K, P>), dim3(blocks_for(c.n, B)), dim3(B), 0, s, b.posrho[1], b.velp[1], \
+                           b.skey, b.range, c);                                                              \
+        return hipGetLastError();                                                                            \
+    }
+    WC_VARIANTS(WC_LAUNCH)
+#undef WC_LAUNCH
+    return hipErrorInvalidValue;
 }
 
-hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, hipStream_t s) {
+hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, int v, hipStream_t s) {
     if (c.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_update, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.posrho[1], b.velp[1],
-                       b.skey, b.range, b.nlist, b.ncount, b.posrho[0], b.velp[0], c, dt);
-    return hipGetLastError();
+    const PassVariant pv = variant(v);
+#define WC_LAUNCH(B, C, K, P)                                                                                 \
+    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                         \
+        hipLaunchKernelGGL((k_update<B, C, K, P>), dim3(blocks_for(c.n, B)), dim3(B), 0, s, b.posrho[1], b.velp[1], \
+                           b.skey, b.range, b.posrho[0], b.velp[0], c, dt);                                  \
+        return hipGetLastError();                                                                            \
+    }
+    WC_VARIANTS(WC_LAUNCH)
+#undef WC_LAUNCH
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n, hipStream_t s) {

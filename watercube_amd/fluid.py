@@ -117,6 +117,21 @@ class Fluid:
         L.check(L.lib().wc_profile_steps(self.ctx, C.c_float(dt), nsteps, ms), "wc_profile_steps")
         return {L.lib().wc_pass_name(k).decode(): ms[k] for k in range(L.WC_NUM_PASSES)}
 
+    def tune(self, dt: float, reps: int = 5) -> dict:
+        """Time every variant of the density and force passes on the current state and keep the
+        fastest (``wc_tune``); the particle state is left as it was."""
+        nv = L.lib().wc_num_variants()
+        md = (C.c_double * nv)()
+        mu = (C.c_double * nv)()
+        t = L.wc_tuning()
+        L.check(L.lib().wc_tune(self.ctx, C.c_float(dt), reps, C.byref(t), md, mu), "wc_tune")
+        return {"density_variant": t.density_variant, "update_variant": t.update_variant,
+                "density_ms": list(md), "update_ms": list(mu), "variants": L.variants()}
+
+    def set_tuning(self, density_variant: int, update_variant: int) -> None:
+        t = L.wc_tuning(density_variant, update_variant)
+        L.check(L.lib().wc_set_tuning(self.ctx, C.byref(t)), "wc_set_tuning")
+
     def close(self) -> None:
         if self.ctx is not None:
             L.lib().wc_destroy(self.ctx)

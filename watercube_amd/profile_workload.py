@@ -25,6 +25,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
+    ap.add_argument("--tuning", default="auto",
+                    help="'auto' runs the autotuner first; 'D,U' fixes the density / force-pass variants")
     args = ap.parse_args()
 
     from watercube_amd import Fluid, scenes
@@ -33,6 +35,11 @@ def main() -> int:
     print(f"profile_workload: {args.workload} n={p.num_particles} copy-calibration bytes/dispatch "
           f"read={32 * p.num_particles} write={32 * p.num_particles}", flush=True)
     with Fluid(p, parts) as f:
+        if args.tuning == "auto":
+            f.tune(dt, reps=3)
+        else:
+            d, u = (int(x) for x in args.tuning.split(","))
+            f.set_tuning(d, u)
         f.step(dt, args.warmup)
         f.synchronize()
         t0 = time.perf_counter()
