@@ -80,6 +80,46 @@ def cpu_baseline(p, parts, dt, budget_s: float = 12.0, max_steps: int = 60) -> d
     }
 
 
+def batched_section(steps: int, warmup: int = 10, latency_samples: int = 20) -> dict:
+    """256 independent scenes (log-uniform 4,096..262,144 particles) advanced as one batch:
+    aggregate particle-steps/s over `steps` graph-launched steps, and per-scene step latency
+    (device clock from the step's first pass to the scene's last particle) over single steps."""
+    from watercube_amd import batch
+
+    scenes = batch.scenes_256(seed=0)
+    with batch.SceneBatch(scenes) as b:
+        n = b.total_particles
+        tuning = b.tune(reps=2)
+        b.step(warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.synchronize()
+        elapsed = time.perf_counter() - t0
+        lat = []
+        for _ in range(latency_samples):
+            b.step(1)
+            b.synchronize()
+            lat.append(b.latency_ms())
+    lat = np.concatenate(lat)
+    kinds = {}
+    for _, _, _, k in scenes:
+        kinds[k] = kinds.get(k, 0) + 1
+    return {
+        "workload": "scenes_256",
+        "scenes": len(scenes),
+        "particles": n,
+        "fills": kinds,
+        "steps": steps,
+        "value": n * steps / elapsed,
+        "unit": "particle-steps/s",
+        "ms_per_step": elapsed * 1e3 / steps,
+        "scene_latency_ms_p50": float(np.percentile(lat, 50)),
+        "scene_latency_ms_p99": float(np.percentile(lat, 99)),
+        "tuning": [tuning["density_variant"], tuning["update_variant"]],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +127,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch-steps", type=int, default=500, help="steps of the 256-scene batch (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step runs on a single MI355X)",
@@ -161,6 +202,9 @@ def main() -> int:
             "algorithmic_bytes": algo[dom],
         },
     }
+    if args.batch_steps > 0:
+        out["batched"] = batched_section(args.batch_steps)
+        print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, parts, dt)
     print(json.dumps(out), flush=True)

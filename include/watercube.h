@@ -115,10 +115,9 @@ WC_API int wc_params_derive(wc_params* p);
 WC_API int wc_scene_block(wc_particle* out, uint32_t nx, uint32_t ny, uint32_t nz, const float origin[3],
                           float spacing, float jitter, const float velocity[3], uint64_t seed);
 
-/* Jittered lattice sphere: the lattice points origin-free at (i + 0.5) * spacing inside the sphere
- * of `center`, taken in x-fastest order, until `count` particles are written.  The radius is the
- * smallest lattice-aligned one that holds `count` points.  Returns WC_ERR_INVALID_ARG if the
- * sphere does not fit in [0,size]^3 for size = 1. */
+/* Jittered lattice ball: of the lattice points (i + 0.5) * spacing of the unit cube, the `count`
+ * nearest to `center` (ties broken by lattice index), written in x-fastest lattice order with
+ * jitter and `velocity`.  Returns WC_ERR_INVALID_ARG if such a ball leaves the unit cube. */
 WC_API int wc_scene_sphere(wc_particle* out, uint32_t count, const float center[3], float spacing, float jitter,
                            const float velocity[3], uint64_t seed);
 
@@ -146,7 +145,8 @@ WC_API int wc_set_params(wc_ctx* ctx, const wc_params* p);
 WC_API int wc_get_params(const wc_ctx* ctx, wc_params* out);
 
 /* Copy n records into particle buffer 1 (the buffer Fluid::update sorts from and the update
- * pass writes to: Fluid.cpp:146, 347-350).  Both buffers are filled, like Fluid.cpp:146 and 155. */
+ * pass writes to: Fluid.cpp:146, 347-350).  Both buffers are filled, like Fluid.cpp:146 and 155.
+ * n is the context's particle count (all scenes, concatenated, for a batch). */
 WC_API int wc_upload(wc_ctx* ctx, const wc_particle* host, uint32_t n);
 
 /* Copy n records out of particle buffer `which` (1 or 2); waits for the stream.  Stands in for
@@ -162,12 +162,35 @@ WC_API int wc_sort_run(wc_ctx* ctx);
 WC_API int wc_density(wc_ctx* ctx);
 
 /* Fluid::runUpdateProg(buffer2, buffer1, dt) (src/core/Fluid.cpp:294-321): update.comp:134-232.
- * `dt` is the integration step itself (the reference passes frame time * time_scale_). */
+ * `dt` is the integration step itself (the reference passes frame time * time_scale_).  Reads
+ * the densities and pressures of the preceding wc_density on the same sort, as the reference's
+ * pass order does. */
 WC_API int wc_update(wc_ctx* ctx, float dt);
 
 /* nsteps of Fluid::update (src/core/Fluid.cpp:342-354): sort, density, update, replayed as one
  * captured HIP graph per step.  Asynchronous: returns once the work is queued. */
 WC_API int wc_step(wc_ctx* ctx, float dt, uint32_t nsteps);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched scenes.  A context may hold many independent scenes that advance together: every pass
+ * is one launch over all scenes (particles and cell grids concatenated in HBM behind a
+ * per-scene table of constants), and a step is one captured graph.  No reference counterpart:
+ * the reference simulates a single fluid (src/core/Scene.cpp:52-56).  wc_upload / wc_download
+ * address the concatenation of all scenes; the cell and order getters above address scene 0.
+ * ------------------------------------------------------------------------------------------ */
+WC_API int wc_create_batch(const wc_params* scenes, uint32_t num_scenes, wc_ctx** out);
+WC_API int wc_num_scenes(const wc_ctx* ctx);
+WC_API int wc_scene_upload(wc_ctx* ctx, uint32_t scene, const wc_particle* host, uint32_t n);
+WC_API int wc_scene_download(wc_ctx* ctx, uint32_t scene, int which, wc_particle* host, uint32_t n);
+/* counts and offsets (relative to the scene's first particle) of one scene's R^3 cells; either
+ * array may be NULL */
+WC_API int wc_scene_get_cells(wc_ctx* ctx, uint32_t scene, uint32_t* counts, uint32_t* offsets, uint32_t ncells);
+/* Per-scene step sizes used by wc_step_scenes (wc_step sets every scene to its dt). */
+WC_API int wc_set_scene_dt(wc_ctx* ctx, const float* dt, uint32_t num_scenes);
+WC_API int wc_step_scenes(wc_ctx* ctx, uint32_t nsteps);
+/* For the last step: milliseconds from the start of its first pass until the scene's last
+ * particle left the force pass (device wall clock), one entry per scene. */
+WC_API int wc_scene_latency(wc_ctx* ctx, double* ms, uint32_t num_scenes);
 
 /* Wait for the context's stream; reports a device-side failure flag as WC_ERR_DEVICE_FAULT. */
 WC_API int wc_synchronize(wc_ctx* ctx);
@@ -200,7 +223,8 @@ WC_API int wc_device_view_get(wc_ctx* ctx, wc_device_view* out);
 
 /* Timing of each kernel of the step, measured with HIP events recorded on the context's stream
  * around every launch (eager launches, no graph).  Runs nsteps full steps and returns in
- * ms_total[k] the summed duration of kernel k over them; wc_pass_name(k) names kernel k. */
+ * ms_total[k] the summed duration of kernel k over them; wc_pass_name(k) names kernel k.
+ * dt > 0 sets every scene's step size first; dt <= 0 keeps the per-scene ones. */
 #define WC_NUM_PASSES 6
 WC_API int wc_profile_steps(wc_ctx* ctx, float dt, uint32_t nsteps, double ms_total[WC_NUM_PASSES]);
 WC_API const char* wc_pass_name(int k);
@@ -221,10 +245,87 @@ WC_API int wc_set_tuning(wc_ctx* ctx, const wc_tuning* tuning);
 
 /* Autotuner: sorts the current state once, then times every variant of the density pass and of
  * the force pass (reps launches each, HIP events on the context's stream) and keeps the fastest
- * of each for later steps.  Particle buffer 1 is restored afterwards; buffer 2 (the sorted copy
+ * of each for later steps.  dt > 0 sets every scene's step size first; dt <= 0 keeps them.  Particle buffer 1 is restored afterwards; buffer 2 (the sorted copy
  * the next step rebuilds anyway) is left sorted.  ms_density / ms_update, when not NULL, receive
  * wc_num_variants() average launch times in milliseconds. */
 WC_API int wc_tune(wc_ctx* ctx, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update);
+
+/* ------------------------------------------------------------------------------------------
+ * Learned closure.  No reference counterpart: the reference has a single explicit step
+ * (src/core/Fluid.cpp:342-354).  A small network predicts, per particle, the correction to the
+ * acceleration that lets one step of K * dt follow K plain steps of dt; the plain step remains
+ * the judge and the fallback.
+ *
+ * Network: 195 inputs -> 128 -> 128 -> 128 -> 3 (SiLU), bf16 weights and activations, fp32
+ * accumulation on matrix cores.  Inputs per particle (bf16, rows of WC_MLP_IN_PAD values, the
+ * last 13 zero): for each of its 32 nearest neighbours within the kernel radius (ascending
+ * distance, missing ones zero) (x_j - x_i) / h and (v_j - v_i) / (1 m/s); then
+ * 10 (rho_i / rho0 - 1), K / 4, and the filled fraction of the 32 places.  Output: the
+ * acceleration correction in units of 9.81 m/s^2 (rows of 4 floats, the 4th zero).
+ * ------------------------------------------------------------------------------------------ */
+#define WC_MLP_IN 195
+#define WC_MLP_IN_PAD 208
+#define WC_MLP_HIDDEN 128
+#define WC_MLP_OUT 3
+typedef struct wc_mlp wc_mlp;
+
+/* Network with Glorot-uniform weights drawn from SplitMix64(seed); max_batch bounds training
+ * batches.  Parameters are one fp32 vector of wc_mlp_num_params() values: W1 [208][128], b1
+ * [128], W2 [128][128], b2, W3 [128][128], b3, W4 [128][32], b4 [32] (row-major [input][output];
+ * rows 195.. of W1 and columns 3.. of W4 / b4 are padding and stay zero). */
+WC_API int wc_mlp_create(uint32_t max_batch, uint64_t seed, wc_mlp** out);
+WC_API int wc_mlp_destroy(wc_mlp* net);
+WC_API int wc_mlp_num_params(void);
+WC_API int wc_mlp_get_params(wc_mlp* net, float* host, uint32_t n);
+WC_API int wc_mlp_set_params(wc_mlp* net, const float* host, uint32_t n);
+/* Inference with the fused kernel: x_dev n rows of WC_MLP_IN_PAD bf16, y_dev n rows of 4 floats. */
+WC_API int wc_mlp_predict(wc_mlp* net, const void* x_dev, void* y_dev, uint32_t n);
+/* Same network through the training path (layer-by-layer GEMMs); n <= max_batch. */
+WC_API int wc_mlp_forward(wc_mlp* net, const void* x_dev, void* y_dev, uint32_t n);
+/* Loss = mean over rows and the 3 outputs of (y - t)^2 for targets t_dev (n rows of 4 floats);
+ * writes the gradient (wc_mlp_num_params() floats) and the loss. */
+WC_API int wc_mlp_gradient(wc_mlp* net, const void* x_dev, const void* t_dev, uint32_t n, float* grad_host,
+                           double* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; loss may be NULL. */
+WC_API int wc_mlp_train_step(wc_mlp* net, const void* x_dev, const void* t_dev, uint32_t n, float lr, double* loss);
+/* Average duration of the fused inference kernel over reps launches (HIP events). */
+WC_API int wc_mlp_profile_predict(wc_mlp* net, const void* x_dev, void* y_dev, uint32_t n, uint32_t reps, double* ms);
+
+/* Device memory for the closure's inputs, targets and outputs. */
+WC_API int wc_device_alloc(size_t bytes, void** out);
+WC_API int wc_device_free(void* dev);
+WC_API int wc_device_upload(void* dev, const void* host, size_t bytes);
+WC_API int wc_device_download(void* host, const void* dev, size_t bytes);
+
+/* Particle identities: once enabled, ids of buffer 1 start as 0..n-1 in its current order and
+ * follow every particle through sorts, so runs can be compared particle by particle. */
+WC_API int wc_track_ids(wc_ctx* ctx, int enable);
+WC_API int wc_get_ids(wc_ctx* ctx, int which, uint32_t* host, uint32_t n);
+/* Largest density written by the last density pass. */
+WC_API int wc_max_density(wc_ctx* ctx, float* out);
+
+/* Closure inputs of the sorted particles of buffer 2 (after wc_sort_run and wc_density): rows
+ * lists the slots to describe (NULL: all n, in slot order); x_dev receives nrows (or n) rows. */
+WC_API int wc_closure_features(wc_ctx* ctx, float dt_mult, const uint32_t* rows, uint32_t nrows, void* x_dev);
+/* Force pass buffer 2 -> buffer 1 with step dt; corr_dev (n rows of 4 floats, times corr_scale)
+ * is added to the acceleration of particles at least 2h from every wall (NULL: none); acc_dev
+ * (NULL: not written) receives each slot's plain acceleration (4th float: 1 where a correction
+ * applies). */
+WC_API int wc_closure_update(wc_ctx* ctx, float dt, const void* corr_dev, float corr_scale, void* acc_dev);
+
+typedef struct wc_closure_stats {
+    uint32_t macro_steps;    /* learned steps of K * dt taken */
+    uint32_t fallbacks;      /* of them replaced by K plain steps */
+    double wall_ms;          /* device time of the whole run */
+    double max_density_jump; /* largest rise of the density maximum over one learned step, / rho0 */
+} wc_closure_stats;
+
+/* nmacro learned steps of K * dt.  Each: sort, density, closure inputs, network, force pass with
+ * the correction.  Judge: after every learned step the new state's density maximum may exceed
+ * the one before the step by at most judge_tol * rho0; otherwise the step is replaced by K plain
+ * steps of dt from the saved state. */
+WC_API int wc_closure_run(wc_ctx* ctx, wc_mlp* net, float dt, uint32_t K, uint32_t nmacro, float judge_tol,
+                          wc_closure_stats* stats);
 
 /* Text of the last failure on the calling thread ("" if none). */
 WC_API const char* wc_last_error(void);

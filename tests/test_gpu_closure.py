@@ -1,0 +1,184 @@
+"""GPU checks of the learned closure: network kernels against the CPU restatement
+(oracle/mlp_oracle.py), the feature kernel against a numpy 32-nearest-neighbour reference, and
+the closure step's plain limits (zero network = plain large steps; a judge that always
+rejects = the plain small-dt trajectory), bit for bit."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+from watercube_amd import Fluid, scenes
+from watercube_amd import _lib as L
+from watercube_amd.closure import ClosureNet, DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_bits(a):
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _random_batch(rng, m):
+    x = np.zeros((m, 208), np.float32)
+    x[:, :195] = rng.normal(0, 1, (m, 195))
+    xb = _bf16_bits(x)
+    t = np.zeros((m, 4), np.float32)
+    t[:, :3] = rng.normal(0, 0.5, (m, 3))
+    return xb, t
+
+
+def test_network_forward_and_gradient_match_cpu():
+    rng = np.random.default_rng(0)
+    m = 300
+    net = ClosureNet(max_batch=m, seed=3)
+    p = net.params().astype(np.float64)
+    p[MO.layout()["b1"][0]:MO.layout()["b1"][0] + 128] = rng.normal(0, 0.1, 128)  # nonzero biases too
+    net.set_params(p.astype(np.float32))
+    xb, t = _random_batch(rng, m)
+    xd = DeviceArray((m, 208), np.uint16).upload(xb)
+    td = DeviceArray((m, 4), np.float32).upload(t)
+    yd = DeviceArray((m, 4), np.float32)
+    X = MO.bf16_bits_to_f64(xb)
+    y_ref, loss_ref, g_ref = MO.forward_backward(p.astype(np.float32).astype(np.float64), X, t[:, :3], bf16=True)
+    net.forward(xd, yd, m)
+    y_gemm = yd.download()[:, :3]
+    net.predict(xd, yd, m)
+    y_fused = yd.download()[:, :3]
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"forward rel err gemm {rel(y_gemm, y_ref):.2e} fused {rel(y_fused, y_ref):.2e}", flush=True)
+    assert rel(y_gemm, y_ref) < 1e-2
+    assert rel(y_fused, y_ref) < 1e-2
+    g, loss = net.gradient(xd, td, m)
+    assert abs(loss - loss_ref) / loss_ref < 1e-2
+    Lo = MO.layout()
+    for k in ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4"):
+        off, n = Lo[k]
+        e = rel(g[off:off + n], g_ref[off:off + n])
+        print(f"grad {k}: rel err {e:.2e}", flush=True)
+        assert e < 3e-2, k
+    # padding receives no gradient
+    w1 = g[Lo["w1"][0]:Lo["w1"][0] + Lo["w1"][1]].reshape(208, 128)
+    assert np.all(w1[195:] == 0)
+
+
+def test_training_fits_a_teacher_network():
+    rng = np.random.default_rng(1)
+    m = 4096
+    teacher = ClosureNet(max_batch=m, seed=11)
+    student = ClosureNet(max_batch=m, seed=12)
+    xb, t = _random_batch(rng, m)
+    xd = DeviceArray((m, 208), np.uint16).upload(xb)
+    yd = DeviceArray((m, 4), np.float32)
+    teacher.forward(xd, yd, m)
+    td = DeviceArray((m, 4), np.float32).upload(yd.download())
+    _, first = student.gradient(xd, td, m)
+    for i in range(300):
+        student.train_step(xd.ptr, td.ptr, m, 3e-3)
+    _, last = student.gradient(xd, td, m)
+    print(f"teacher fit: loss {first:.4f} -> {last:.5f}", flush=True)
+    assert last < 0.1 * first
+
+
+def _enumeration(oracle, p, pos, counts, offsets, t):
+    R = p.grid_res
+    cx, cy, cz = oracle.cell(p, pos[t])
+    out = []
+    for b in range(27):
+        nx, ny, nz = cx + b // 9 - 1, cy + (b // 3) % 3 - 1, cz + b % 3 - 1
+        if min(nx, ny, nz) < 0 or max(nx, ny, nz) >= R:
+            continue
+        k = (nz * R + ny) * R + nx
+        out += list(range(offsets[k], offsets[k] + counts[k]))
+    return out
+
+
+def test_features_match_numpy_reference(oracle):
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    K = 4.0
+    with Fluid(p, parts) as f:
+        f.step(dt, 30)
+        f.sort_.run()
+        f.runDensityProg()
+        s = f.particles(2)
+        counts, offsets = f.sort_.getCountBuffer(), f.sort_.getOffsetBuffer()
+        rows = np.arange(0, p.num_particles, 37, dtype=np.uint32)
+        xd = DeviceArray((len(rows), 208), np.uint16)
+        L.check(L.lib().wc_closure_features(f.ctx, C.c_float(K), L.u32ptr(rows), len(rows), xd.ptr), "features")
+        got = xd.download()
+    f32 = np.float32
+    h = f32(p.kernel_radius)
+    cut2 = f32(h * h)
+    while np.sqrt(np.nextafter(cut2, f32(0))) >= h:
+        cut2 = np.nextafter(cut2, f32(0))
+    while np.sqrt(cut2) < h:
+        cut2 = np.nextafter(cut2, f32(np.inf))
+    inv_h = f32(1.0) / h
+    pos, vel, rho = s["position"], s["velocity"], s["density"]
+    for r, t in enumerate(rows):
+        cand = []
+        for e, u in enumerate(_enumeration(oracle, p, pos, counts, offsets, t)):
+            if u == t:
+                continue
+            d = pos[t] - pos[u]
+            d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+            if not (d2 >= cut2):
+                cand.append((d2, e, u))
+        cand.sort(key=lambda c: (c[0], c[1]))
+        cand = cand[:32]
+        want = np.zeros(208, np.float32)
+        for k, (_, _, u) in enumerate(cand):
+            want[6 * k:6 * k + 3] = (pos[u] - pos[t]) * inv_h
+            want[6 * k + 3:6 * k + 6] = (vel[u] - vel[t]) * f32(1.0)
+        want[192] = (rho[t] / f32(p.rest_density) - f32(1.0)) * f32(10.0)
+        want[193] = f32(K) * f32(0.25)
+        want[194] = f32(len(cand)) * f32(1.0 / 32)
+        assert np.array_equal(got[r], _bf16_bits(want)), f"row {r} (slot {t}) differs"
+
+
+def _zero_net():
+    net = ClosureNet(max_batch=64)
+    net.set_params(np.zeros(ClosureNet.num_params(), np.float32))
+    return net
+
+
+def test_zero_network_equals_plain_large_steps():
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    K, macro = 4, 10
+    with Fluid(p, parts) as a:
+        a.step(dt * K, macro)
+        want = a.particles(1)
+    net = _zero_net()  # keep the handle alive for the whole run
+    with Fluid(p, parts) as b:
+        st = L.wc_closure_stats()
+        L.check(L.lib().wc_closure_run(b.ctx, net.h, C.c_float(dt), K, macro, C.c_float(1e9), C.byref(st)),
+                "wc_closure_run")
+        assert st.macro_steps == macro and st.fallbacks == 0
+        assert b.particles(1).tobytes() == want.tobytes()
+
+
+def test_rejecting_judge_reproduces_plain_small_steps(oracle):
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    K, macro = 4, 6
+    ref = oracle.OracleState(p, parts)
+    ref.step(dt, K * macro)
+    net = _zero_net()
+    with Fluid(p, parts) as b:
+        st = L.wc_closure_stats()
+        L.check(L.lib().wc_closure_run(b.ctx, net.h, C.c_float(dt), K, macro, C.c_float(-1.0), C.byref(st)),
+                "wc_closure_run")
+        assert st.fallbacks == macro
+        assert b.particles(1).tobytes() == ref.buf1.tobytes()
+
+
+def test_identities_follow_particles():
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    with Fluid(p, parts) as f:
+        f.track_ids(True)
+        f.step(dt, 7)
+        ids = f.ids(1)
+        assert np.array_equal(np.sort(ids), np.arange(p.num_particles))
+        f.sort_.run()
+        order = f.sort_.getSortedBuffer()
+        assert np.array_equal(f.ids(2), ids[order])

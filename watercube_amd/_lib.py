@@ -97,6 +97,11 @@ class wc_device_view(C.Structure):
     ]
 
 
+class wc_closure_stats(C.Structure):
+    _fields_ = [("macro_steps", C.c_uint32), ("fallbacks", C.c_uint32), ("wall_ms", C.c_double),
+                ("max_density_jump", C.c_double)]
+
+
 class wc_tuning(C.Structure):
     _fields_ = [("density_variant", C.c_int32), ("update_variant", C.c_int32)]
 
@@ -124,6 +129,14 @@ SIGNATURES = {
     "wc_update": (C.c_int, [_vp, C.c_float]),
     "wc_step": (C.c_int, [_vp, C.c_float, C.c_uint32]),
     "wc_synchronize": (C.c_int, [_vp]),
+    "wc_create_batch": (C.c_int, [_P(wc_params), C.c_uint32, _P(_vp)]),
+    "wc_num_scenes": (C.c_int, [_vp]),
+    "wc_scene_upload": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint32]),
+    "wc_scene_download": (C.c_int, [_vp, C.c_uint32, C.c_int, _vp, C.c_uint32]),
+    "wc_scene_get_cells": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.c_uint32]),
+    "wc_set_scene_dt": (C.c_int, [_vp, _f32p, C.c_uint32]),
+    "wc_step_scenes": (C.c_int, [_vp, C.c_uint32]),
+    "wc_scene_latency": (C.c_int, [_vp, _P(C.c_double), C.c_uint32]),
     "wc_get_counts": (C.c_int, [_vp, _u32p, C.c_uint32]),
     "wc_get_offsets": (C.c_int, [_vp, _u32p, C.c_uint32]),
     "wc_get_cell_keys": (C.c_int, [_vp, _u32p, C.c_uint32]),
@@ -136,6 +149,26 @@ SIGNATURES = {
     "wc_get_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
     "wc_set_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
     "wc_tune": (C.c_int, [_vp, C.c_float, C.c_uint32, _P(wc_tuning), _P(C.c_double), _P(C.c_double)]),
+    "wc_mlp_create": (C.c_int, [C.c_uint32, C.c_uint64, _P(_vp)]),
+    "wc_mlp_destroy": (C.c_int, [_vp]),
+    "wc_mlp_num_params": (C.c_int, []),
+    "wc_mlp_get_params": (C.c_int, [_vp, _f32p, C.c_uint32]),
+    "wc_mlp_set_params": (C.c_int, [_vp, _f32p, C.c_uint32]),
+    "wc_mlp_predict": (C.c_int, [_vp, _vp, _vp, C.c_uint32]),
+    "wc_mlp_forward": (C.c_int, [_vp, _vp, _vp, C.c_uint32]),
+    "wc_mlp_gradient": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _f32p, _P(C.c_double)]),
+    "wc_mlp_train_step": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_float, _P(C.c_double)]),
+    "wc_mlp_profile_predict": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _P(C.c_double)]),
+    "wc_device_alloc": (C.c_int, [C.c_size_t, _P(_vp)]),
+    "wc_device_free": (C.c_int, [_vp]),
+    "wc_device_upload": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "wc_device_download": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "wc_track_ids": (C.c_int, [_vp, C.c_int]),
+    "wc_get_ids": (C.c_int, [_vp, C.c_int, _u32p, C.c_uint32]),
+    "wc_max_density": (C.c_int, [_vp, _f32p]),
+    "wc_closure_features": (C.c_int, [_vp, C.c_float, _u32p, C.c_uint32, _vp]),
+    "wc_closure_update": (C.c_int, [_vp, C.c_float, _vp, C.c_float, _vp]),
+    "wc_closure_run": (C.c_int, [_vp, _vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, _P(wc_closure_stats)]),
     "wc_last_error": (C.c_char_p, []),
     "wc_abi_version": (C.c_int, []),
     "wc_target_arch": (C.c_char_p, []),

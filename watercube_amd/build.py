@@ -19,7 +19,7 @@ LIB = PKG / "libwatercube.so"
 HARNESS = PKG / "wc_harness"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["wc_api.hip", "wc_kernels.hip", "wc_scenes.cpp"]
+SOURCES = ["wc_api.hip", "wc_kernels.hip", "wc_mlp.hip", "wc_scenes.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

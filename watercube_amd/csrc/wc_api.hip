@@ -1,5 +1,6 @@
-// wc_api.hip -- host runtime behind include/watercube.h: parameters, contexts, device buffers,
-// the captured per-step graph and per-kernel event timing.
+// wc_api.hip -- host runtime behind include/watercube.h: parameters, contexts of one or many
+// scenes, device buffers, the captured per-step graph, per-kernel event timing and the
+// autotuner of the neighbour passes.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "wc_internal.hpp"
+#include "wc_mlp.hpp"
 
 namespace {
 
@@ -32,6 +34,8 @@ int fail(int code, const char* fmt, ...) {
         hipError_t e_ = (call);                                                                        \
         if (e_ != hipSuccess) return fail(WC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
+
+uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
 
@@ -91,23 +95,41 @@ StepConsts make_consts(const wc_params& p) {
 }  // namespace wc
 
 using wc::DeviceBuffers;
-using wc::StepConsts;
+using wc::SceneDesc;
 
 struct wc_ctx {
-    wc_params params;
-    StepConsts consts;
-    uint32_t n_cap = 0;
-    uint32_t cells_cap = 0;
+    std::vector<wc_params> params;   // per scene
+    std::vector<SceneDesc> desc;     // per scene, mirrored in d_desc
+    std::vector<uint32_t> tables;    // blk64 | blk128 | blk256 | tiles, each num_scenes + 1
+    uint32_t num_scenes = 0;
+    uint32_t total_particles = 0;
+    uint32_t total_cells = 0;        // padded to scan tiles per scene
+    uint32_t particle_cap = 0;       // allocated particle slots (single-scene contexts may shrink)
+    uint32_t cell_cap = 0;
+    uint32_t tile_cap = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     DeviceBuffers b{};
-    wc_particle* staging = nullptr;  // n_cap records for upload / download
+    SceneDesc* d_desc = nullptr;
+    uint32_t* d_tables = nullptr;
+    wc_particle* staging = nullptr;  // particle_cap records for upload / download
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    float graph_dt = 0.0f;
     bool graph_valid = false;
     int density_variant = 2;  // (64 threads, 32 list entries, 4 in flight, per-plane drain) until tuned
     int update_variant = 2;
+    double clock_khz = 100000.0;  // wall_clock64 rate
+    // learned-closure workspace (allocated on first use)
+    uint16_t* cl_x = nullptr;       // features, n x 208 bf16
+    float* cl_y = nullptr;          // network output, n x 4
+    float4* cl_backup[2] = {nullptr, nullptr};
+    uint32_t* cl_backup_ids = nullptr;
+    uint32_t* cl_rows = nullptr;    // row list for feature requests
+    uint32_t cl_rows_cap = 0;
+};
+
+struct wc_mlp {
+    wc::Mlp m;
 };
 
 namespace {
@@ -122,16 +144,80 @@ int check_params(const wc_params* p) {
     return WC_OK;
 }
 
-int free_ctx_buffers(wc_ctx* c) {
+// Lays the scenes out (particle and cell offsets, block and tile tables) from c->params.
+int layout_scenes(wc_ctx* c) {
+    const uint32_t S = c->num_scenes;
+    c->desc.assign(S, SceneDesc{});
+    c->tables.assign(4 * (size_t)(S + 1), 0u);
+    uint32_t* blk64 = &c->tables[0];
+    uint32_t* blk128 = blk64 + (S + 1);
+    uint32_t* blk256 = blk128 + (S + 1);
+    uint32_t* tiles = blk256 + (S + 1);
+    uint64_t p_off = 0, c_off = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        const wc_params& p = c->params[s];
+        SceneDesc& d = c->desc[s];
+        d.c = wc::make_consts(p);
+        d.p_off = (uint32_t)p_off;
+        d.c_off = (uint32_t)c_off;
+        const uint32_t nt = ceil_div(d.c.ncells, wc::kScanTile);
+        blk64[s + 1] = blk64[s] + ceil_div(d.c.n, 64);
+        blk128[s + 1] = blk128[s] + ceil_div(d.c.n, 128);
+        blk256[s + 1] = blk256[s] + ceil_div(d.c.n, 256);
+        tiles[s + 1] = tiles[s] + nt;
+        p_off += d.c.n;
+        c_off += (uint64_t)nt * wc::kScanTile;
+        if (p_off > 0xFFFFFF00ull || c_off > 0xFFFFFF00ull)
+            return fail(WC_ERR_CAPACITY, "scenes exceed 2^32 particles or cells in one context");
+    }
+    c->total_particles = (uint32_t)p_off;
+    c->total_cells = (uint32_t)c_off;
+    return WC_OK;
+}
+
+void fill_scene_views(wc_ctx* c) {
+    const uint32_t S = c->num_scenes;
+    wc::Scenes& v = c->b.sc;
+    v.desc = c->d_desc;
+    v.blk64 = c->d_tables;
+    v.blk128 = c->d_tables + (S + 1);
+    v.blk256 = c->d_tables + 2 * (S + 1);
+    v.tiles = c->d_tables + 3 * (S + 1);
+    v.num_scenes = S;
+    v.total_particles = c->total_particles;
+    v.total_tiles = c->tables[4 * (size_t)(S + 1) - 1];
+    v.blocks64 = c->tables[S];
+    v.blocks128 = c->tables[2 * (size_t)(S + 1) - 1];
+    v.blocks256 = c->tables[3 * (size_t)(S + 1) - 1];
+}
+
+int upload_tables(wc_ctx* c) {
+    WC_HIP(hipMemcpyAsync(c->d_desc, c->desc.data(), c->desc.size() * sizeof(SceneDesc), hipMemcpyHostToDevice,
+                          c->stream));
+    WC_HIP(hipMemcpyAsync(c->d_tables, c->tables.data(), c->tables.size() * 4, hipMemcpyHostToDevice, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    fill_scene_views(c);
+    return WC_OK;
+}
+
+void free_ctx_buffers(wc_ctx* c) {
     DeviceBuffers& b = c->b;
-    void* ptrs[] = {b.posrho[0], b.posrho[1], b.velp[0], b.velp[1], b.key,         b.rank,
-                    b.slot_tmp,  b.sorted,    b.skey,    b.counts,  b.range,       b.scan_status,
-                    b.scan_ticket, b.error_flag, c->staging};
+    void* ptrs[] = {b.posrho[0],   b.posrho[1],   b.velp[0],     b.velp[1],     b.key,       b.rank,
+                    b.slot_tmp,    b.sorted,      b.skey,        b.counts,      b.range,     b.scan_status,
+                    b.scan_ticket, b.error_flag,  b.scene_done,  b.step_start,  c->staging,  c->d_desc,
+                    c->d_tables,   b.rho_max,     b.ids[0],      b.ids[1],      c->cl_x,     c->cl_y,
+                    c->cl_backup[0], c->cl_backup[1], c->cl_backup_ids, c->cl_rows};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     memset(&b, 0, sizeof(b));
     c->staging = nullptr;
-    return WC_OK;
+    c->d_desc = nullptr;
+    c->d_tables = nullptr;
+    c->cl_x = nullptr;
+    c->cl_y = nullptr;
+    c->cl_backup[0] = c->cl_backup[1] = nullptr;
+    c->cl_backup_ids = nullptr;
+    c->cl_rows = nullptr;
 }
 
 void drop_graph(wc_ctx* c) {
@@ -142,19 +228,29 @@ void drop_graph(wc_ctx* c) {
     c->graph_valid = false;
 }
 
+void destroy_ctx(wc_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    free_ctx_buffers(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
 int enqueue_sort(wc_ctx* c) {
-    WC_HIP(wc::launch_hash(c->b, c->consts, c->stream));
-    WC_HIP(wc::launch_scan(c->b, c->consts, c->stream));
-    WC_HIP(wc::launch_place(c->b, c->consts, c->stream));
-    WC_HIP(wc::launch_gather(c->b, c->consts, c->stream));
+    WC_HIP(wc::launch_hash(c->b, c->stream));
+    WC_HIP(wc::launch_scan(c->b, c->stream));
+    WC_HIP(wc::launch_place(c->b, c->stream));
+    WC_HIP(wc::launch_gather(c->b, c->stream));
     return WC_OK;
 }
 
-int enqueue_step(wc_ctx* c, float dt) {
+int enqueue_step(wc_ctx* c) {
     int rc = enqueue_sort(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_density(c->b, c->consts, c->density_variant, c->stream));
-    WC_HIP(wc::launch_update(c->b, c->consts, dt, c->update_variant, c->stream));
+    WC_HIP(wc::launch_density(c->b, c->density_variant, c->stream));
+    WC_HIP(wc::launch_update(c->b, c->update_variant, c->stream));
     return WC_OK;
 }
 
@@ -162,6 +258,215 @@ int set_device(const wc_ctx* c) {
     WC_HIP(hipSetDevice(c->device));
     return WC_OK;
 }
+
+int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
+    if (!out) return fail(WC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!scenes || num == 0) return fail(WC_ERR_INVALID_ARG, "no scenes");
+    for (uint32_t s = 0; s < num; ++s) {
+        int rc = check_params(&scenes[s]);
+        if (rc) return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WC_ERR_NO_DEVICE, "no HIP device");
+    wc_ctx* c = new (std::nothrow) wc_ctx();
+    if (!c) return fail(WC_ERR_NO_MEMORY, "host allocation failed");
+    c->num_scenes = num;
+    c->params.assign(scenes, scenes + num);
+    int rc = layout_scenes(c);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    c->particle_cap = c->total_particles;
+    c->cell_cap = c->total_cells;
+    c->tile_cap = c->tables.back();
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return fail(WC_ERR_NO_DEVICE, "hipGetDevice failed");
+    }
+    int rate = 0;
+    if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && rate > 0)
+        c->clock_khz = rate;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(WC_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
+    const size_t n = c->particle_cap ? c->particle_cap : 1;
+    const size_t cells = c->cell_cap ? c->cell_cap : 1;
+    const size_t tiles = c->tile_cap ? c->tile_cap : 1;
+    DeviceBuffers& b = c->b;
+    bool ok = true;
+    auto alloc = [&](void** ptr, size_t bytes) {
+        if (!ok) return;
+        if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) ok = false;
+    };
+    alloc((void**)&b.posrho[0], n * sizeof(float4));
+    alloc((void**)&b.posrho[1], n * sizeof(float4));
+    alloc((void**)&b.velp[0], n * sizeof(float4));
+    alloc((void**)&b.velp[1], n * sizeof(float4));
+    alloc((void**)&b.key, n * 4);
+    alloc((void**)&b.rank, n * 4);
+    alloc((void**)&b.slot_tmp, n * 4);
+    alloc((void**)&b.sorted, n * 4);
+    alloc((void**)&b.skey, n * 4);
+    alloc((void**)&b.counts, cells * 4 + 64);
+    alloc((void**)&b.range, cells * 8 + 64);
+    alloc((void**)&b.scan_status, tiles * 8 + 8);
+    alloc((void**)&b.scan_ticket, 64);
+    alloc((void**)&b.error_flag, 64);
+    alloc((void**)&b.scene_done, (size_t)num * 8 + 8);
+    alloc((void**)&b.step_start, 64);
+    alloc((void**)&b.rho_max, 64);
+    alloc((void**)&c->staging, n * sizeof(wc_particle));
+    alloc((void**)&c->d_desc, (size_t)num * sizeof(SceneDesc));
+    alloc((void**)&c->d_tables, c->tables.size() * 4);
+    if (!ok) {
+        destroy_ctx(c);
+        return fail(WC_ERR_NO_MEMORY, "device allocation failed (%u particles, %u cells)", c->total_particles,
+                    c->total_cells);
+    }
+    hipError_t e = hipSuccess;
+    auto zero = [&](void* p, size_t bytes) {
+        if (e == hipSuccess) e = hipMemsetAsync(p, 0, bytes, c->stream);
+    };
+    zero(b.counts, cells * 4 + 64);
+    zero(b.range, cells * 8 + 64);
+    zero(b.scan_status, tiles * 8 + 8);
+    zero(b.error_flag, 64);
+    zero(b.scene_done, (size_t)num * 8 + 8);
+    zero(b.step_start, 64);
+    zero(b.rho_max, 64);
+    zero(b.sorted, n * 4);
+    zero(b.skey, n * 4);
+    zero(b.key, n * 4);
+    zero(b.slot_tmp, n * 4);
+    zero(b.posrho[0], n * sizeof(float4));
+    zero(b.posrho[1], n * sizeof(float4));
+    zero(b.velp[0], n * sizeof(float4));
+    zero(b.velp[1], n * sizeof(float4));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        destroy_ctx(c);
+        return fail(WC_ERR_HIP, "buffer initialisation failed: %s", hipGetErrorString(e));
+    }
+    rc = upload_tables(c);
+    if (rc) {
+        destroy_ctx(c);
+        return rc;
+    }
+    *out = c;
+    return WC_OK;
+}
+
+int capture_step(wc_ctx* c) {
+    drop_graph(c);
+    WC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(WC_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    c->graph = g;
+    e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(c);
+        return fail(WC_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    c->graph_valid = true;
+    return WC_OK;
+}
+
+int run_steps(wc_ctx* c, uint32_t nsteps) {
+    if (nsteps == 0) return WC_OK;
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (!c->graph_valid) {
+        rc = capture_step(c);
+        if (rc) return rc;
+    }
+    for (uint32_t s = 0; s < nsteps; ++s) WC_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    return WC_OK;
+}
+
+// Sets every scene's dt.  Only the scene table changes (host copy and device copy); captured
+// graphs read dt from the device table, so they stay valid.
+int set_all_dt(wc_ctx* c, const float* dt, bool same) {
+    bool changed = false;
+    for (uint32_t s = 0; s < c->num_scenes; ++s) {
+        const float v = same ? dt[0] : dt[s];
+        if (c->desc[s].dt != v) {
+            c->desc[s].dt = v;
+            changed = true;
+        }
+    }
+    if (!changed) return WC_OK;
+    int rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(hipMemcpyAsync(c->d_desc, c->desc.data(), c->desc.size() * sizeof(SceneDesc), hipMemcpyHostToDevice,
+                          c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));  // the host table may change again before the copy runs
+    return WC_OK;
+}
+
+int copy_records_in(wc_ctx* c, uint32_t first, const wc_particle* host, uint32_t n) {
+    if (n == 0) return WC_OK;
+    int rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(hipMemcpyAsync(c->staging + first, host, (size_t)n * sizeof(wc_particle), hipMemcpyHostToDevice,
+                          c->stream));
+    for (int b = 0; b < 2; ++b)
+        WC_HIP(wc::launch_split_records(c->staging + first, c->b.posrho[b] + first, c->b.velp[b] + first, n,
+                                        c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+int copy_records_out(wc_ctx* c, int which, uint32_t first, wc_particle* host, uint32_t n) {
+    if (which != 1 && which != 2) return fail(WC_ERR_INVALID_ARG, "buffer must be 1 or 2, got %d", which);
+    if (n == 0) return WC_OK;
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int b = which - 1;
+    WC_HIP(wc::launch_join_records(c->b.posrho[b] + first, c->b.velp[b] + first, c->staging + first, n, c->stream));
+    WC_HIP(hipMemcpyAsync(host, c->staging + first, (size_t)n * sizeof(wc_particle), hipMemcpyDeviceToHost,
+                          c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+int scene_cells(wc_ctx* c, uint32_t scene, uint32_t* counts, uint32_t* offsets, uint32_t ncells) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
+    if (scene >= c->num_scenes) return fail(WC_ERR_INVALID_ARG, "scene %u of %u", scene, c->num_scenes);
+    const SceneDesc& d = c->desc[scene];
+    if (ncells != d.c.ncells) return fail(WC_ERR_INVALID_ARG, "expected %u cells, got %u", d.c.ncells, ncells);
+    int rc = set_device(c);
+    if (rc) return rc;
+    std::vector<uint32_t> tmp((size_t)ncells * 2);
+    WC_HIP(hipStreamSynchronize(c->stream));
+    if (ncells) WC_HIP(hipMemcpy(tmp.data(), c->b.range + d.c_off, (size_t)ncells * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ncells; ++i) {
+        if (counts) counts[i] = tmp[2 * i + 1];
+        if (offsets) offsets[i] = tmp[2 * i] - d.p_off;
+    }
+    return WC_OK;
+}
+
+int download_u32(wc_ctx* c, const uint32_t* dev, uint32_t* host, uint32_t n) {
+    if (!host && n) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (n > c->total_particles) return fail(WC_ERR_CAPACITY, "%u entries requested of %u", n, c->total_particles);
+    if (n == 0) return WC_OK;
+    int rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(hipStreamSynchronize(c->stream));
+    WC_HIP(hipMemcpy(host, dev, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return WC_OK;
+}
+
+const char* const kPassNames[WC_NUM_PASSES] = {"hash", "scan", "place", "gather", "density", "update"};
 
 }  // namespace
 
@@ -241,137 +546,79 @@ WC_API int wc_params_preset(wc_params* p, int preset) {
 }
 
 WC_API int wc_create(const wc_params* p, wc_ctx** out) {
-    if (!out) return fail(WC_ERR_INVALID_ARG, "out is NULL");
-    *out = nullptr;
-    int rc = check_params(p);
-    if (rc) return rc;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WC_ERR_NO_DEVICE, "no HIP device");
-    wc_ctx* c = new (std::nothrow) wc_ctx();
-    if (!c) return fail(WC_ERR_NO_MEMORY, "host allocation failed");
-    c->params = *p;
-    c->consts = wc::make_consts(*p);
-    c->n_cap = p->num_particles;
-    c->cells_cap = c->consts.ncells;
-    if (hipGetDevice(&c->device) != hipSuccess) {
-        delete c;
-        return fail(WC_ERR_NO_DEVICE, "hipGetDevice failed");
+    if (!p) {
+        if (out) *out = nullptr;
+        return fail(WC_ERR_INVALID_ARG, "params is NULL");
     }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return fail(WC_ERR_HIP, "hipStreamCreateWithFlags failed");
-    }
-    const size_t n = c->n_cap ? c->n_cap : 1;
-    const size_t cells = c->cells_cap;
-    const uint32_t tiles = (uint32_t)((cells + wc::kScanTile - 1) / wc::kScanTile);
-    DeviceBuffers& b = c->b;
-    b.num_scan_tiles = tiles;
-    bool ok = true;
-    auto alloc = [&](void** ptr, size_t bytes) {
-        if (!ok) return;
-        if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) ok = false;
-    };
-    alloc((void**)&b.posrho[0], n * sizeof(float4));
-    alloc((void**)&b.posrho[1], n * sizeof(float4));
-    alloc((void**)&b.velp[0], n * sizeof(float4));
-    alloc((void**)&b.velp[1], n * sizeof(float4));
-    alloc((void**)&b.key, n * 4);
-    alloc((void**)&b.rank, n * 4);
-    alloc((void**)&b.slot_tmp, n * 4);
-    alloc((void**)&b.sorted, n * 4);
-    alloc((void**)&b.skey, n * 4);
-    alloc((void**)&b.counts, cells * 4 + 64);
-    alloc((void**)&b.range, cells * 8 + 64);
-    alloc((void**)&b.scan_status, (size_t)tiles * 8 + 8);
-    alloc((void**)&b.scan_ticket, 64);
-    alloc((void**)&b.error_flag, 64);
-    alloc((void**)&c->staging, n * sizeof(wc_particle));
-    if (!ok) {
-        free_ctx_buffers(c);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
-        return fail(WC_ERR_NO_MEMORY, "device allocation failed (%u particles, %zu cells)", p->num_particles, cells);
-    }
-    hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = hipMemsetAsync(b.counts, 0, cells * 4 + 64, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.range, 0, cells * 8 + 64, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.error_flag, 0, 64, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.sorted, 0, n * 4, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.skey, 0, n * 4, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.key, 0, n * 4, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.posrho[0], 0, n * sizeof(float4), c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.posrho[1], 0, n * sizeof(float4), c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.velp[0], 0, n * sizeof(float4), c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b.velp[1], 0, n * sizeof(float4), c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        free_ctx_buffers(c);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
-        return fail(WC_ERR_HIP, "buffer initialisation failed: %s", hipGetErrorString(e));
-    }
-    *out = c;
-    return WC_OK;
+    return create_context(p, 1, out);
+}
+
+WC_API int wc_create_batch(const wc_params* scenes, uint32_t num_scenes, wc_ctx** out) {
+    return create_context(scenes, num_scenes, out);
 }
 
 WC_API int wc_destroy(wc_ctx* c) {
-    if (!c) return WC_OK;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    drop_graph(c);
-    free_ctx_buffers(c);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
+    destroy_ctx(c);
     return WC_OK;
 }
+
+WC_API int wc_num_scenes(const wc_ctx* c) { return c ? (int)c->num_scenes : fail(WC_ERR_INVALID_ARG, "ctx is NULL"); }
 
 WC_API int wc_set_params(wc_ctx* c, const wc_params* p) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     int rc = check_params(p);
     if (rc) return rc;
+    if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "wc_set_params needs a single-scene context");
     const uint64_t R = (uint64_t)p->grid_res;
-    if (p->num_particles > c->n_cap || R * R * R > c->cells_cap)
+    const uint64_t padded = (R * R * R + wc::kScanTile - 1) / wc::kScanTile * wc::kScanTile;
+    if (p->num_particles > c->particle_cap || padded > c->cell_cap)
         return fail(WC_ERR_CAPACITY, "num_particles %u / cells %llu exceed the context's %u / %u", p->num_particles,
-                    (unsigned long long)(R * R * R), c->n_cap, c->cells_cap);
-    c->params = *p;
-    c->consts = wc::make_consts(*p);
+                    (unsigned long long)(R * R * R), c->particle_cap, c->cell_cap);
+    rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(hipStreamSynchronize(c->stream));
+    const float dt = c->desc[0].dt;
+    c->params[0] = *p;
+    rc = layout_scenes(c);
+    if (rc) return rc;
+    c->desc[0].dt = dt;
     drop_graph(c);
-    return WC_OK;
+    return upload_tables(c);
 }
 
 WC_API int wc_get_params(const wc_ctx* c, wc_params* out) {
     if (!c || !out) return fail(WC_ERR_INVALID_ARG, "NULL argument");
-    *out = c->params;
+    *out = c->params[0];
     return WC_OK;
 }
 
 WC_API int wc_upload(wc_ctx* c, const wc_particle* host, uint32_t n) {
     if (!c || (!host && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
-    if (n != c->params.num_particles)
-        return fail(WC_ERR_INVALID_ARG, "upload of %u records into a context of %u particles", n,
-                    c->params.num_particles);
-    if (n == 0) return WC_OK;
-    int rc = set_device(c);
-    if (rc) return rc;
-    WC_HIP(hipMemcpyAsync(c->staging, host, (size_t)n * sizeof(wc_particle), hipMemcpyHostToDevice, c->stream));
-    WC_HIP(wc::launch_split_records(c->staging, c->b.posrho[0], c->b.velp[0], n, c->stream));
-    WC_HIP(wc::launch_split_records(c->staging, c->b.posrho[1], c->b.velp[1], n, c->stream));
-    WC_HIP(hipStreamSynchronize(c->stream));
-    return WC_OK;
+    if (n != c->total_particles)
+        return fail(WC_ERR_INVALID_ARG, "upload of %u records into a context of %u particles", n, c->total_particles);
+    return copy_records_in(c, 0, host, n);
 }
 
 WC_API int wc_download(wc_ctx* c, int which, wc_particle* host, uint32_t n) {
     if (!c || (!host && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
-    if (which != 1 && which != 2) return fail(WC_ERR_INVALID_ARG, "buffer must be 1 or 2, got %d", which);
-    if (n > c->params.num_particles) return fail(WC_ERR_CAPACITY, "download of %u records from %u", n, c->params.num_particles);
-    if (n == 0) return WC_OK;
-    int rc = set_device(c);
-    if (rc) return rc;
-    const int b = which - 1;
-    WC_HIP(wc::launch_join_records(c->b.posrho[b], c->b.velp[b], c->staging, n, c->stream));
-    WC_HIP(hipMemcpyAsync(host, c->staging, (size_t)n * sizeof(wc_particle), hipMemcpyDeviceToHost, c->stream));
-    WC_HIP(hipStreamSynchronize(c->stream));
-    return WC_OK;
+    if (n > c->total_particles) return fail(WC_ERR_CAPACITY, "download of %u records from %u", n, c->total_particles);
+    return copy_records_out(c, which, 0, host, n);
+}
+
+WC_API int wc_scene_upload(wc_ctx* c, uint32_t scene, const wc_particle* host, uint32_t n) {
+    if (!c || (!host && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (scene >= c->num_scenes) return fail(WC_ERR_INVALID_ARG, "scene %u of %u", scene, c->num_scenes);
+    if (n != c->desc[scene].c.n)
+        return fail(WC_ERR_INVALID_ARG, "scene %u holds %u particles, got %u", scene, c->desc[scene].c.n, n);
+    return copy_records_in(c, c->desc[scene].p_off, host, n);
+}
+
+WC_API int wc_scene_download(wc_ctx* c, uint32_t scene, int which, wc_particle* host, uint32_t n) {
+    if (!c || (!host && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (scene >= c->num_scenes) return fail(WC_ERR_INVALID_ARG, "scene %u of %u", scene, c->num_scenes);
+    if (n > c->desc[scene].c.n)
+        return fail(WC_ERR_CAPACITY, "scene %u holds %u particles, asked %u", scene, c->desc[scene].c.n, n);
+    return copy_records_out(c, which, c->desc[scene].p_off, host, n);
 }
 
 WC_API int wc_sort_run(wc_ctx* c) {
@@ -385,45 +632,37 @@ WC_API int wc_density(wc_ctx* c) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     int rc = set_device(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_density(c->b, c->consts, c->density_variant, c->stream));
+    WC_HIP(wc::launch_density(c->b, c->density_variant, c->stream));
     return WC_OK;
 }
 
 WC_API int wc_update(wc_ctx* c, float dt) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
-    int rc = set_device(c);
+    int rc = set_all_dt(c, &dt, true);
     if (rc) return rc;
-    WC_HIP(wc::launch_update(c->b, c->consts, dt, c->update_variant, c->stream));
+    rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(wc::launch_update(c->b, c->update_variant, c->stream));
     return WC_OK;
 }
 
 WC_API int wc_step(wc_ctx* c, float dt, uint32_t nsteps) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
-    if (nsteps == 0) return WC_OK;
-    int rc = set_device(c);
+    int rc = set_all_dt(c, &dt, true);
     if (rc) return rc;
-    if (!c->graph_valid || c->graph_dt != dt) {
-        drop_graph(c);
-        WC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_step(c, dt);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return fail(WC_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
-        c->graph = g;
-        e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            drop_graph(c);
-            return fail(WC_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-        }
-        c->graph_dt = dt;
-        c->graph_valid = true;
-    }
-    for (uint32_t s = 0; s < nsteps; ++s) WC_HIP(hipGraphLaunch(c->graph_exec, c->stream));
-    return WC_OK;
+    return run_steps(c, nsteps);
+}
+
+WC_API int wc_set_scene_dt(wc_ctx* c, const float* dt, uint32_t num_scenes) {
+    if (!c || !dt) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (num_scenes != c->num_scenes)
+        return fail(WC_ERR_INVALID_ARG, "%u step sizes for %u scenes", num_scenes, c->num_scenes);
+    return set_all_dt(c, dt, false);
+}
+
+WC_API int wc_step_scenes(wc_ctx* c, uint32_t nsteps) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
+    return run_steps(c, nsteps);
 }
 
 WC_API int wc_synchronize(wc_ctx* c) {
@@ -437,31 +676,25 @@ WC_API int wc_synchronize(wc_ctx* c) {
     return WC_OK;
 }
 
-static int download_range(wc_ctx* c, uint32_t* host, uint32_t ncells, int field) {
-    if (!c || (!host && ncells)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
-    if (ncells != c->consts.ncells)
-        return fail(WC_ERR_INVALID_ARG, "expected %u cells, got %u", c->consts.ncells, ncells);
+WC_API int wc_scene_latency(wc_ctx* c, double* ms, uint32_t num_scenes) {
+    if (!c || !ms) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (num_scenes != c->num_scenes)
+        return fail(WC_ERR_INVALID_ARG, "%u entries for %u scenes", num_scenes, c->num_scenes);
     int rc = set_device(c);
     if (rc) return rc;
-    std::vector<uint32_t> tmp((size_t)ncells * 2);
+    std::vector<unsigned long long> done(num_scenes);
+    unsigned long long start = 0;
     WC_HIP(hipStreamSynchronize(c->stream));
-    WC_HIP(hipMemcpy(tmp.data(), c->b.range, (size_t)ncells * 8, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < ncells; ++i) host[i] = tmp[2 * i + field];
+    WC_HIP(hipMemcpy(done.data(), c->b.scene_done, (size_t)num_scenes * 8, hipMemcpyDeviceToHost));
+    WC_HIP(hipMemcpy(&start, c->b.step_start, 8, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < num_scenes; ++s) ms[s] = done[s] > start ? (double)(done[s] - start) / c->clock_khz : 0.0;
     return WC_OK;
 }
 
-WC_API int wc_get_counts(wc_ctx* c, uint32_t* host, uint32_t ncells) { return download_range(c, host, ncells, 1); }
-WC_API int wc_get_offsets(wc_ctx* c, uint32_t* host, uint32_t ncells) { return download_range(c, host, ncells, 0); }
-
-static int download_u32(wc_ctx* c, const uint32_t* dev, uint32_t* host, uint32_t n) {
-    if (!c || (!host && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
-    if (n > c->params.num_particles) return fail(WC_ERR_CAPACITY, "%u entries requested of %u", n, c->params.num_particles);
-    if (n == 0) return WC_OK;
-    int rc = set_device(c);
-    if (rc) return rc;
-    WC_HIP(hipStreamSynchronize(c->stream));
-    WC_HIP(hipMemcpy(host, dev, (size_t)n * 4, hipMemcpyDeviceToHost));
-    return WC_OK;
+WC_API int wc_get_counts(wc_ctx* c, uint32_t* host, uint32_t ncells) { return scene_cells(c, 0, host, nullptr, ncells); }
+WC_API int wc_get_offsets(wc_ctx* c, uint32_t* host, uint32_t ncells) { return scene_cells(c, 0, nullptr, host, ncells); }
+WC_API int wc_scene_get_cells(wc_ctx* c, uint32_t scene, uint32_t* counts, uint32_t* offsets, uint32_t ncells) {
+    return scene_cells(c, scene, counts, offsets, ncells);
 }
 
 WC_API int wc_get_cell_keys(wc_ctx* c, uint32_t* host, uint32_t n) {
@@ -480,12 +713,10 @@ WC_API int wc_device_view_get(wc_ctx* c, wc_device_view* out) {
     out->cell_range = c->b.range;
     out->sorted = c->b.sorted;
     out->stream = (void*)c->stream;
-    out->num_particles = c->params.num_particles;
-    out->num_cells = c->consts.ncells;
+    out->num_particles = c->total_particles;
+    out->num_cells = c->total_cells;
     return WC_OK;
 }
-
-static const char* const kPassNames[WC_NUM_PASSES] = {"hash", "scan", "place", "gather", "density", "update"};
 
 WC_API const char* wc_pass_name(int k) { return (k >= 0 && k < WC_NUM_PASSES) ? kPassNames[k] : ""; }
 
@@ -493,7 +724,9 @@ WC_API int wc_profile_steps(wc_ctx* c, float dt, uint32_t nsteps, double ms_tota
     if (!c || !ms_total) return fail(WC_ERR_INVALID_ARG, "NULL argument");
     for (int k = 0; k < WC_NUM_PASSES; ++k) ms_total[k] = 0.0;
     if (nsteps == 0) return WC_OK;
-    int rc = set_device(c);
+    int rc = dt > 0.0f ? set_all_dt(c, &dt, true) : WC_OK;
+    if (rc) return rc;
+    rc = set_device(c);
     if (rc) return rc;
     const size_t nev = (size_t)nsteps * (WC_NUM_PASSES + 1);
     std::vector<hipEvent_t> ev(nev, nullptr);
@@ -507,17 +740,17 @@ WC_API int wc_profile_steps(wc_ctx* c, float dt, uint32_t nsteps, double ms_tota
     for (uint32_t s = 0; s < nsteps && err == hipSuccess; ++s) {
         hipEvent_t* e = &ev[(size_t)s * (WC_NUM_PASSES + 1)];
         err = hipEventRecord(e[0], c->stream);
-        if (err == hipSuccess) err = wc::launch_hash(c->b, c->consts, c->stream);
+        if (err == hipSuccess) err = wc::launch_hash(c->b, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[1], c->stream);
-        if (err == hipSuccess) err = wc::launch_scan(c->b, c->consts, c->stream);
+        if (err == hipSuccess) err = wc::launch_scan(c->b, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[2], c->stream);
-        if (err == hipSuccess) err = wc::launch_place(c->b, c->consts, c->stream);
+        if (err == hipSuccess) err = wc::launch_place(c->b, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
-        if (err == hipSuccess) err = wc::launch_gather(c->b, c->consts, c->stream);
+        if (err == hipSuccess) err = wc::launch_gather(c->b, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[4], c->stream);
-        if (err == hipSuccess) err = wc::launch_density(c->b, c->consts, c->density_variant, c->stream);
+        if (err == hipSuccess) err = wc::launch_density(c->b, c->density_variant, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[5], c->stream);
-        if (err == hipSuccess) err = wc::launch_update(c->b, c->consts, dt, c->update_variant, c->stream);
+        if (err == hipSuccess) err = wc::launch_update(c->b, c->update_variant, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[6], c->stream);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
@@ -538,7 +771,8 @@ WC_API int wc_profile_steps(wc_ctx* c, float dt, uint32_t nsteps, double ms_tota
 WC_API int wc_num_variants(void) { return wc::num_variants(); }
 
 WC_API int wc_variant_info(int v, int32_t* block, int32_t* list_cap, int32_t* chunk, int32_t* plane_drain) {
-    if (v < 0 || v >= wc::num_variants()) return fail(WC_ERR_INVALID_ARG, "variant %d outside [0, %d)", v, wc::num_variants());
+    if (v < 0 || v >= wc::num_variants())
+        return fail(WC_ERR_INVALID_ARG, "variant %d outside [0, %d)", v, wc::num_variants());
     const wc::PassVariant pv = wc::variant(v);
     if (block) *block = pv.block;
     if (list_cap) *list_cap = pv.list_cap;
@@ -568,10 +802,12 @@ WC_API int wc_set_tuning(wc_ctx* c, const wc_tuning* t) {
 WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     if (reps == 0) reps = 1;
-    int rc = set_device(c);
+    int rc = dt > 0.0f ? set_all_dt(c, &dt, true) : WC_OK;
+    if (rc) return rc;
+    rc = set_device(c);
     if (rc) return rc;
     const int nv = wc::num_variants();
-    const size_t n = c->params.num_particles;
+    const size_t n = c->total_particles;
     std::vector<double> td(nv, 0.0), tu(nv, 0.0);
     if (n > 0) {
         // Keep buffer 1 (the state the caller sees); the passes below rewrite it.
@@ -584,16 +820,17 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
         hipEvent_t e0 = nullptr, e1 = nullptr;
         hipError_t err = hipEventCreate(&e0);
         if (err == hipSuccess) err = hipEventCreate(&e1);
-        if (err == hipSuccess) err = hipMemcpyAsync(save_pos, c->b.posrho[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
-        if (err == hipSuccess) err = hipMemcpyAsync(save_vel, c->b.velp[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(save_pos, c->b.posrho[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(save_vel, c->b.velp[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
         if (err == hipSuccess) err = (enqueue_sort(c) == WC_OK) ? hipSuccess : hipErrorUnknown;
         // Both passes are idempotent on the sorted buffer: density rewrites the same densities,
         // the force pass rewrites buffer 1 from buffer 2 alone.
         for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
             for (int v = 0; v < nv && err == hipSuccess; ++v) {
                 auto launch = [&]() {
-                    return pass == 0 ? wc::launch_density(c->b, c->consts, v, c->stream)
-                                     : wc::launch_update(c->b, c->consts, dt, v, c->stream);
+                    return pass == 0 ? wc::launch_density(c->b, v, c->stream) : wc::launch_update(c->b, v, c->stream);
                 };
                 err = launch();  // warm-up
                 if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
@@ -604,10 +841,11 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
                 if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
                 (pass == 0 ? td : tu)[v] = (double)ms / reps;
             }
-            if (pass == 0 && err == hipSuccess) err = wc::launch_density(c->b, c->consts, c->density_variant, c->stream);
         }
-        if (err == hipSuccess) err = hipMemcpyAsync(c->b.posrho[0], save_pos, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
-        if (err == hipSuccess) err = hipMemcpyAsync(c->b.velp[0], save_vel, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(c->b.posrho[0], save_pos, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(c->b.velp[0], save_vel, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
         if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -632,6 +870,419 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
         if (ms_update) ms_update[v] = tu[v];
     }
     return WC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Device memory helpers (for data the closure's training and inference read from HBM).
+// ------------------------------------------------------------------------------------------
+WC_API int wc_device_alloc(size_t bytes, void** out) {
+    if (!out) return fail(WC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (hipMalloc(out, bytes ? bytes : 16) != hipSuccess) return fail(WC_ERR_NO_MEMORY, "hipMalloc of %zu bytes", bytes);
+    return WC_OK;
+}
+WC_API int wc_device_free(void* p) {
+    if (p) (void)hipFree(p);
+    return WC_OK;
+}
+WC_API int wc_device_upload(void* dev, const void* host, size_t bytes) {
+    if ((!dev || !host) && bytes) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (bytes) WC_HIP(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    return WC_OK;
+}
+WC_API int wc_device_download(void* host, const void* dev, size_t bytes) {
+    if ((!dev || !host) && bytes) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (bytes) WC_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+    return WC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Particle identities and the density maximum.
+// ------------------------------------------------------------------------------------------
+WC_API int wc_track_ids(wc_ctx* c, int enable) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
+    int rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(hipStreamSynchronize(c->stream));
+    if (!enable) {
+        for (int b = 0; b < 2; ++b) {
+            if (c->b.ids[b]) (void)hipFree(c->b.ids[b]);
+            c->b.ids[b] = nullptr;
+        }
+        drop_graph(c);
+        return WC_OK;
+    }
+    const size_t n = c->particle_cap ? c->particle_cap : 1;
+    for (int b = 0; b < 2; ++b)
+        if (!c->b.ids[b] && hipMalloc(&c->b.ids[b], n * 4) != hipSuccess)
+            return fail(WC_ERR_NO_MEMORY, "identity buffers");
+    std::vector<uint32_t> iota(n);
+    for (size_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
+    for (int b = 0; b < 2; ++b) WC_HIP(hipMemcpy(c->b.ids[b], iota.data(), n * 4, hipMemcpyHostToDevice));
+    drop_graph(c);  // the captured kernels now carry the identity buffers
+    return WC_OK;
+}
+
+WC_API int wc_get_ids(wc_ctx* c, int which, uint32_t* host, uint32_t n) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
+    if (which != 1 && which != 2) return fail(WC_ERR_INVALID_ARG, "buffer must be 1 or 2");
+    if (!c->b.ids[0]) return fail(WC_ERR_INVALID_ARG, "identities are not tracked (wc_track_ids)");
+    return download_u32(c, c->b.ids[which - 1], host, n);
+}
+
+WC_API int wc_max_density(wc_ctx* c, float* out) {
+    if (!c || !out) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    uint32_t bits = 0;
+    WC_HIP(hipStreamSynchronize(c->stream));
+    WC_HIP(hipMemcpy(&bits, c->b.rho_max, 4, hipMemcpyDeviceToHost));
+    memcpy(out, &bits, 4);
+    return WC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// The closure's network.
+// ------------------------------------------------------------------------------------------
+WC_API int wc_mlp_num_params(void) { return (int)wc::mlp_layout().total; }
+
+WC_API int wc_mlp_create(uint32_t max_batch, uint64_t seed, wc_mlp** out) {
+    if (!out) return fail(WC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WC_ERR_NO_DEVICE, "no HIP device");
+    wc_mlp* h = new (std::nothrow) wc_mlp();
+    if (!h) return fail(WC_ERR_NO_MEMORY, "host allocation failed");
+    wc::Mlp& m = h->m;
+    m.max_batch = max_batch ? max_batch : 1;
+    (void)hipGetDevice(&m.device);
+    const wc::MlpLayout L = wc::mlp_layout();
+    const size_t B = m.max_batch;
+    bool ok = hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking) == hipSuccess;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (ok && hipMalloc(p, bytes ? bytes : 16) != hipSuccess) ok = false;
+    };
+    alloc((void**)&m.master, L.total * 4);
+    alloc((void**)&m.grad, L.total * 4);
+    alloc((void**)&m.adam_m, L.total * 4);
+    alloc((void**)&m.adam_v, L.total * 4);
+    alloc((void**)&m.w1, (size_t)wc::kMlpInPad * wc::kMlpHidden * 2);
+    alloc((void**)&m.w1t, (size_t)wc::kMlpInPad * wc::kMlpHidden * 2);
+    alloc((void**)&m.w2, (size_t)wc::kMlpHidden * wc::kMlpHidden * 2);
+    alloc((void**)&m.w2t, (size_t)wc::kMlpHidden * wc::kMlpHidden * 2);
+    alloc((void**)&m.w3, (size_t)wc::kMlpHidden * wc::kMlpHidden * 2);
+    alloc((void**)&m.w3t, (size_t)wc::kMlpHidden * wc::kMlpHidden * 2);
+    alloc((void**)&m.w4, (size_t)wc::kMlpHidden * wc::kMlpOutPad * 2);
+    alloc((void**)&m.w4t, (size_t)wc::kMlpHidden * wc::kMlpOutPad * 2);
+    for (int l = 0; l < 3; ++l) {
+        alloc((void**)&m.z[l], B * wc::kMlpHidden * 4);
+        alloc((void**)&m.h[l], B * wc::kMlpHidden * 2);
+    }
+    alloc((void**)&m.yacc, B * wc::kMlpOutPad * 4);
+    alloc((void**)&m.d4, B * wc::kMlpOutPad * 2);
+    alloc((void**)&m.gbuf, B * wc::kMlpHidden * 4);
+    alloc((void**)&m.d, B * wc::kMlpHidden * 2);
+    alloc((void**)&m.loss, 64);
+    if (!ok) {
+        wc_mlp_destroy(h);
+        return fail(WC_ERR_NO_MEMORY, "network buffers for a batch of %u", max_batch);
+    }
+    // Glorot-uniform weights from SplitMix64, zero biases; padded rows and columns stay zero.
+    std::vector<float> w(L.total, 0.0f);
+    uint64_t s = seed;
+    auto next = [&]() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    auto fill = [&](size_t off, int rows, int cols, int used_rows, int used_cols) {
+        const double a = sqrt(6.0 / (double)(used_rows + used_cols));
+        for (int r = 0; r < rows; ++r)
+            for (int col = 0; col < cols; ++col) {
+                const double u = (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+                if (r < used_rows && col < used_cols) w[off + (size_t)r * cols + col] = (float)((2.0 * u - 1.0) * a);
+            }
+    };
+    fill(L.w1, wc::kMlpInPad, wc::kMlpHidden, wc::kMlpIn, wc::kMlpHidden);
+    fill(L.w2, wc::kMlpHidden, wc::kMlpHidden, wc::kMlpHidden, wc::kMlpHidden);
+    fill(L.w3, wc::kMlpHidden, wc::kMlpHidden, wc::kMlpHidden, wc::kMlpHidden);
+    fill(L.w4, wc::kMlpHidden, wc::kMlpOutPad, wc::kMlpHidden, wc::kMlpOut);
+    hipError_t e = hipMemcpy(m.master, w.data(), L.total * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(m.adam_m, 0, L.total * 4);
+    if (e == hipSuccess) e = hipMemset(m.adam_v, 0, L.total * 4);
+    if (e == hipSuccess) e = wc::mlp_refresh_bf16(m, m.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m.stream);
+    if (e != hipSuccess) {
+        wc_mlp_destroy(h);
+        return fail(WC_ERR_HIP, "network initialisation: %s", hipGetErrorString(e));
+    }
+    *out = h;
+    return WC_OK;
+}
+
+WC_API int wc_mlp_destroy(wc_mlp* h) {
+    if (!h) return WC_OK;
+    wc::Mlp& m = h->m;
+    if (m.stream) (void)hipStreamSynchronize(m.stream);
+    void* ptrs[] = {m.master, m.grad, m.adam_m, m.adam_v, m.w1, m.w1t, m.w2, m.w2t, m.w3, m.w3t, m.w4, m.w4t,
+                    m.z[0], m.z[1], m.z[2], m.h[0], m.h[1], m.h[2], m.yacc, m.d4, m.gbuf, m.d, m.loss};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (m.stream) (void)hipStreamDestroy(m.stream);
+    delete h;
+    return WC_OK;
+}
+
+WC_API int wc_mlp_get_params(wc_mlp* h, float* host, uint32_t n) {
+    if (!h || !host) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (n != wc::mlp_layout().total) return fail(WC_ERR_INVALID_ARG, "expected %zu parameters", wc::mlp_layout().total);
+    WC_HIP(hipStreamSynchronize(h->m.stream));
+    WC_HIP(hipMemcpy(host, h->m.master, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return WC_OK;
+}
+
+WC_API int wc_mlp_set_params(wc_mlp* h, const float* host, uint32_t n) {
+    if (!h || !host) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (n != wc::mlp_layout().total) return fail(WC_ERR_INVALID_ARG, "expected %zu parameters", wc::mlp_layout().total);
+    WC_HIP(hipStreamSynchronize(h->m.stream));
+    WC_HIP(hipMemcpy(h->m.master, host, (size_t)n * 4, hipMemcpyHostToDevice));
+    WC_HIP(hipMemset(h->m.adam_m, 0, (size_t)n * 4));
+    WC_HIP(hipMemset(h->m.adam_v, 0, (size_t)n * 4));
+    h->m.adam_t = 0;
+    WC_HIP(wc::mlp_refresh_bf16(h->m, h->m.stream));
+    WC_HIP(hipStreamSynchronize(h->m.stream));
+    return WC_OK;
+}
+
+WC_API int wc_mlp_predict(wc_mlp* h, const void* x_dev, void* y_dev, uint32_t n) {
+    if (!h || ((!x_dev || !y_dev) && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    WC_HIP(wc::mlp_infer(h->m, (const uint16_t*)x_dev, (float*)y_dev, n, h->m.stream));
+    WC_HIP(hipStreamSynchronize(h->m.stream));
+    return WC_OK;
+}
+
+WC_API int wc_mlp_forward(wc_mlp* h, const void* x_dev, void* y_dev, uint32_t n) {
+    if (!h || ((!x_dev || !y_dev) && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (n > h->m.max_batch) return fail(WC_ERR_CAPACITY, "batch %u above the network's %u", n, h->m.max_batch);
+    if (n == 0) return WC_OK;
+    WC_HIP(wc::mlp_forward_train(h->m, (const uint16_t*)x_dev, nullptr, (float*)y_dev, n, h->m.stream));
+    WC_HIP(hipStreamSynchronize(h->m.stream));
+    return WC_OK;
+}
+
+static int mlp_grad(wc_mlp* h, const void* x_dev, const void* t_dev, uint32_t n, double* loss) {
+    if (!h || !x_dev || !t_dev) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (n == 0 || n > h->m.max_batch) return fail(WC_ERR_CAPACITY, "batch %u outside [1, %u]", n, h->m.max_batch);
+    wc::Mlp& m = h->m;
+    WC_HIP(wc::mlp_forward_train(m, (const uint16_t*)x_dev, (const float*)t_dev, nullptr, n, m.stream));
+    WC_HIP(wc::mlp_backward(m, (const uint16_t*)x_dev, n, m.stream));
+    if (loss) {
+        float sse = 0.0f;
+        WC_HIP(hipMemcpyAsync(&sse, m.loss, 4, hipMemcpyDeviceToHost, m.stream));
+        WC_HIP(hipStreamSynchronize(m.stream));
+        *loss = (double)sse / (3.0 * n);
+    }
+    return WC_OK;
+}
+
+WC_API int wc_mlp_gradient(wc_mlp* h, const void* x_dev, const void* t_dev, uint32_t n, float* grad_host,
+                           double* loss) {
+    int rc = mlp_grad(h, x_dev, t_dev, n, loss);
+    if (rc) return rc;
+    if (grad_host) {
+        WC_HIP(hipStreamSynchronize(h->m.stream));
+        WC_HIP(hipMemcpy(grad_host, h->m.grad, wc::mlp_layout().total * 4, hipMemcpyDeviceToHost));
+    }
+    return WC_OK;
+}
+
+WC_API int wc_mlp_train_step(wc_mlp* h, const void* x_dev, const void* t_dev, uint32_t n, float lr, double* loss) {
+    int rc = mlp_grad(h, x_dev, t_dev, n, loss);
+    if (rc) return rc;
+    WC_HIP(wc::mlp_adam(h->m, lr, h->m.stream));
+    if (!loss) WC_HIP(hipStreamSynchronize(h->m.stream));
+    return WC_OK;
+}
+
+WC_API int wc_mlp_profile_predict(wc_mlp* h, const void* x_dev, void* y_dev, uint32_t n, uint32_t reps, double* ms) {
+    if (!h || !ms || ((!x_dev || !y_dev) && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (reps == 0) reps = 1;
+    hipEvent_t e0, e1;
+    WC_HIP(hipEventCreate(&e0));
+    WC_HIP(hipEventCreate(&e1));
+    hipError_t err = wc::mlp_infer(h->m, (const uint16_t*)x_dev, (float*)y_dev, n, h->m.stream);
+    if (err == hipSuccess) err = hipEventRecord(e0, h->m.stream);
+    for (uint32_t r = 0; r < reps && err == hipSuccess; ++r)
+        err = wc::mlp_infer(h->m, (const uint16_t*)x_dev, (float*)y_dev, n, h->m.stream);
+    if (err == hipSuccess) err = hipEventRecord(e1, h->m.stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return fail(WC_ERR_HIP, "timed inference: %s", hipGetErrorString(err));
+    *ms = (double)t / reps;
+    return WC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Learned closure on a single-scene context.
+// ------------------------------------------------------------------------------------------
+static int closure_workspace(wc_ctx* c) {
+    if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "the learned closure runs on single-scene contexts");
+    const size_t n = c->particle_cap ? c->particle_cap : 1;
+    bool ok = true;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (ok && !*p && hipMalloc(p, bytes) != hipSuccess) ok = false;
+    };
+    alloc((void**)&c->cl_x, n * wc::kClosureFeatures * 2);
+    alloc((void**)&c->cl_y, n * 16);
+    alloc((void**)&c->cl_backup[0], n * sizeof(float4));
+    alloc((void**)&c->cl_backup[1], n * sizeof(float4));
+    if (!ok) return fail(WC_ERR_NO_MEMORY, "closure workspace for %zu particles", n);
+    return WC_OK;
+}
+
+WC_API int wc_closure_features(wc_ctx* c, float dt_mult, const uint32_t* rows_host, uint32_t nrows, void* x_dev) {
+    if (!c || !x_dev) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "the learned closure runs on single-scene contexts");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const uint32_t n = c->total_particles;
+    const uint32_t* rows = nullptr;
+    if (rows_host) {
+        for (uint32_t r = 0; r < nrows; ++r)
+            if (rows_host[r] >= n) return fail(WC_ERR_INVALID_ARG, "row %u names slot %u of %u", r, rows_host[r], n);
+        if (nrows > c->cl_rows_cap) {
+            if (c->cl_rows) (void)hipFree(c->cl_rows);
+            c->cl_rows = nullptr;
+            c->cl_rows_cap = 0;
+            if (hipMalloc(&c->cl_rows, (size_t)nrows * 4) != hipSuccess) return fail(WC_ERR_NO_MEMORY, "row list");
+            c->cl_rows_cap = nrows;
+        }
+        WC_HIP(hipMemcpyAsync(c->cl_rows, rows_host, (size_t)nrows * 4, hipMemcpyHostToDevice, c->stream));
+        rows = c->cl_rows;
+    } else {
+        nrows = n;
+    }
+    WC_HIP(wc::launch_closure_features(c->b, rows, nrows, dt_mult, 1.0f, (uint16_t*)x_dev, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
+WC_API int wc_closure_update(wc_ctx* c, float dt, const void* corr_dev, float corr_scale, void* acc_dev) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
+    if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "the learned closure runs on single-scene contexts");
+    int rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(wc::launch_closure_update(c->b, dt, (const float*)corr_dev, corr_scale, (float*)acc_dev, c->stream));
+    return WC_OK;
+}
+
+static int read_rho_max(wc_ctx* c, float* out) {
+    uint32_t bits = 0;
+    WC_HIP(hipMemcpyAsync(&bits, c->b.rho_max, 4, hipMemcpyDeviceToHost, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    memcpy(out, &bits, 4);
+    return WC_OK;
+}
+
+WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t nmacro, float judge_tol,
+                          wc_closure_stats* stats) {
+    if (!c || !net || !stats) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (K == 0) return fail(WC_ERR_INVALID_ARG, "K must be >= 1");
+    int rc = closure_workspace(c);
+    if (rc) return rc;
+    rc = set_all_dt(c, &dt, true);
+    if (rc) return rc;
+    rc = set_device(c);
+    if (rc) return rc;
+    memset(stats, 0, sizeof(*stats));
+    const uint32_t n = c->total_particles;
+    const float rho0 = c->desc[0].c.rho0;
+    const float dt_large = dt * (float)K;
+    const float g = 9.81f;
+    if (c->b.ids[0] && !c->cl_backup_ids && hipMalloc(&c->cl_backup_ids, (size_t)(n ? n : 1) * 4) != hipSuccess)
+        return fail(WC_ERR_NO_MEMORY, "identity backup");
+    hipEvent_t e0, e1;
+    WC_HIP(hipEventCreate(&e0));
+    WC_HIP(hipEventCreate(&e1));
+    WC_HIP(hipEventRecord(e0, c->stream));
+    float prev_max = 0.0f;
+    // Judge the learned step that produced buffer 1: sort + density of the new state, compare
+    // the density maximum with the one before the step; past the tolerance, replace the step by
+    // K plain steps from the saved state.
+    auto sort_density = [&]() -> int {
+        int r = enqueue_sort(c);
+        if (r) return r;
+        WC_HIP(wc::launch_density(c->b, c->density_variant, c->stream));
+        return WC_OK;
+    };
+    auto judge = [&](float* now) -> int {
+        int r = sort_density();
+        if (r) return r;
+        r = read_rho_max(c, now);
+        if (r) return r;
+        const float jump = (*now - prev_max) / rho0;
+        if ((double)jump > stats->max_density_jump) stats->max_density_jump = jump;
+        if (!(jump <= judge_tol)) {  // NaN fails too
+            stats->fallbacks += 1;
+            WC_HIP(hipMemcpyAsync(c->b.posrho[0], c->cl_backup[0], (size_t)n * sizeof(float4),
+                                  hipMemcpyDeviceToDevice, c->stream));
+            WC_HIP(hipMemcpyAsync(c->b.velp[0], c->cl_backup[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
+                                  c->stream));
+            if (c->b.ids[0])
+                WC_HIP(hipMemcpyAsync(c->b.ids[0], c->cl_backup_ids, (size_t)n * 4, hipMemcpyDeviceToDevice,
+                                      c->stream));
+            r = run_steps(c, K);
+            if (r) return r;
+            r = sort_density();
+            if (r) return r;
+            r = read_rho_max(c, now);
+            if (r) return r;
+        }
+        return WC_OK;
+    };
+    rc = sort_density();
+    if (!rc) rc = read_rho_max(c, &prev_max);
+    for (uint32_t m = 0; m < nmacro && !rc; ++m) {
+        if (m > 0) {
+            float now = 0.0f;
+            rc = judge(&now);
+            if (rc) break;
+            prev_max = now;
+        }
+        // save the state at the start of this step (sorted, with densities) for a fallback
+        hipError_t e = hipMemcpyAsync(c->cl_backup[0], c->b.posrho[1], (size_t)n * sizeof(float4),
+                                      hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->cl_backup[1], c->b.velp[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
+                               c->stream);
+        if (e == hipSuccess && c->b.ids[0])
+            e = hipMemcpyAsync(c->cl_backup_ids, c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(net->m.stream);
+        if (e == hipSuccess) e = wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream);
+        if (e == hipSuccess) e = wc::launch_closure_update(c->b, dt_large, c->cl_y, g, nullptr, c->stream);
+        if (e != hipSuccess) rc = fail(WC_ERR_HIP, "learned step: %s", hipGetErrorString(e));
+        stats->macro_steps += 1;
+    }
+    if (!rc && nmacro > 0) {
+        float now = 0.0f;
+        rc = judge(&now);
+    }
+    if (!rc) {
+        hipError_t e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) rc = fail(WC_ERR_HIP, "closure timing: %s", hipGetErrorString(e));
+        stats->wall_ms = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 }  // extern "C"

@@ -1,4 +1,10 @@
 // wc_internal.hpp -- types shared by the host runtime and the gfx950 kernels of the SPH step.
+//
+// A context holds one or more independent scenes.  Their particles are concatenated in the
+// particle buffers and their cell grids in the cell arrays; a device-side table describes each
+// scene (constants, first particle slot, first cell, step size).  Kernels launch once per pass
+// over all scenes: a workgroup finds its scene by a binary search of the per-scene first-block
+// table, which is uniform across the workgroup and so runs on the scalar unit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,17 +14,17 @@
 
 namespace wc {
 
-constexpr int kBlock = 256;           // threads per workgroup for the per-particle kernels
+constexpr int kHashBlock = 256;       // threads per workgroup: hash, place, gather, copies
 constexpr int kScanThreads = 256;     // scan workgroup
 constexpr int kScanItems = 16;        // counts per scan thread (four 16-byte loads)
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 constexpr uint32_t kLookbackSpinLimit = 1u << 24;  // bounded wait in the scan look-back
 
-// Everything one step's kernels read besides buffers, precomputed on the host in float so the
-// device sees exactly the bits the CPU transcription uses.
+// Per-scene constants, precomputed on the host in float so the device sees exactly the bits the
+// CPU transcription uses.
 struct StepConsts {
-    uint32_t n;          // particles
+    uint32_t n;          // particles of the scene
     uint32_t ncells;     // R^3
     int32_t R;           // grid resolution
     float bin;           // binSize
@@ -41,45 +47,82 @@ struct StepConsts {
     float cam[3], dir[3];
 };
 
+struct SceneDesc {
+    StepConsts c;
+    uint32_t p_off;  // first particle slot of the scene
+    uint32_t c_off;  // first cell of the scene (a multiple of kScanTile)
+    float dt;        // integration step
+    uint32_t pad;
+};
+
+// Device tables of a context's scenes, passed by value to every kernel.
+struct Scenes {
+    const SceneDesc* desc;
+    const uint32_t* blk64;   // first 64-thread workgroup of each scene; num_scenes + 1 entries
+    const uint32_t* blk128;  // same for 128-thread workgroups
+    const uint32_t* blk256;  // same for 256-thread workgroups
+    const uint32_t* tiles;   // first scan tile of each scene; num_scenes + 1 entries
+    uint32_t num_scenes;
+    uint32_t total_particles;
+    uint32_t total_tiles;
+    uint32_t blocks64, blocks128, blocks256;
+};
+
 // Device buffers of one context.  posrho/velp are the 32-byte record split in its two
 // 16-byte halves: (x, y, z, density) and (vx, vy, vz, pressure).
 struct DeviceBuffers {
     float4* posrho[2];
     float4* velp[2];
-    uint32_t* key;       // cell index per particle of buffer 1 (input order)
+    uint32_t* key;       // cell index (scene-local) per particle of buffer 1
     uint32_t* rank;      // arrival rank inside its cell (atomic order)
     uint32_t* slot_tmp;  // slot -> input index in arrival order
     uint32_t* sorted;    // slot -> input index in stable order
-    uint32_t* skey;      // slot -> cell index of the sorted particle
-    uint32_t* counts;    // R^3 atomic histogram; the scan reads and zeroes it
-    uint2* range;        // R^3 (exclusive offset, count)
+    uint32_t* skey;      // slot -> scene-local cell index of the sorted particle
+    uint32_t* counts;    // per cell: atomic histogram; the scan reads and zeroes it
+    uint2* range;        // per cell: (first slot, count); slots are context-global
     unsigned long long* scan_status;  // one look-back word per scan tile
     uint32_t* scan_ticket;            // dynamic tile counter
     uint32_t* error_flag;             // nonzero after a failed bounded wait
-    uint32_t num_scan_tiles;
+    unsigned long long* scene_done;   // per scene: wall clock when its force pass finished
+    unsigned long long* step_start;   // wall clock when the last step's hash pass started
+    uint32_t* rho_max;                // bits of the largest density of the last density pass
+    uint32_t* ids[2];                 // particle identity per slot of buffer 1 / 2 (NULL: not tracked)
+    Scenes sc;
 };
 
-// Kernel launchers (wc_kernels.hip).  Each enqueues on `stream` and returns the launch status.
-hipError_t launch_hash(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
-hipError_t launch_scan(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
-hipError_t launch_place(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
-hipError_t launch_gather(const DeviceBuffers& b, const StepConsts& c, hipStream_t stream);
+// Learned closure (single-scene contexts).
+constexpr int kClosureNeighbours = 32;
+constexpr int kClosureFeatures = 208;  // 195 used, zero padded
+// Features of the sorted particles at `rows` (slots; NULL = every slot) into X (bf16 rows).
+hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows, uint32_t nrows, float dt_mult,
+                                   float v_ref, uint16_t* X, hipStream_t stream);
+// Force pass with dt given directly; corr (4 floats per slot, units of g, may be NULL) is added to
+// the acceleration of particles at least 2h from every wall; acc_out (may be NULL) receives the
+// plain acceleration.
+hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
+                                 float* acc_out, hipStream_t stream);
+
 // Tuning variant of the neighbour passes; all variants produce identical results.
 struct PassVariant {
-    int block;     // threads per workgroup
-    int list_cap;  // LDS list entries per lane
-    int chunk;     // neighbour slots loaded per step of a lane's walk
+    int block;         // threads per workgroup
+    int list_cap;      // LDS list entries per lane
+    int chunk;         // neighbour slots loaded per step of a lane's walk
     bool plane_drain;  // drain the list after every dx plane (else only when full and at the end)
 };
 int num_variants();
 PassVariant variant(int v);
-hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, int variant, hipStream_t stream);
-hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, int variant, hipStream_t stream);
+
+// Kernel launchers (wc_kernels.hip).  Each enqueues on `stream` and returns the launch status.
+hipError_t launch_hash(const DeviceBuffers& b, hipStream_t stream);
+hipError_t launch_scan(const DeviceBuffers& b, hipStream_t stream);
+hipError_t launch_place(const DeviceBuffers& b, hipStream_t stream);
+hipError_t launch_gather(const DeviceBuffers& b, hipStream_t stream);
+hipError_t launch_density(const DeviceBuffers& b, int variant, hipStream_t stream);
+hipError_t launch_update(const DeviceBuffers& b, int variant, hipStream_t stream);
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n,
                                 hipStream_t stream);
 hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,
                                hipStream_t stream);
-hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 
 StepConsts make_consts(const wc_params& p);
 float dist2_cutoff(float h);

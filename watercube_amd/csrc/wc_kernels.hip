@@ -1,5 +1,6 @@
 // wc_kernels.hip -- gfx950 kernels of the SPH step: grid build (hash + histogram, decoupled
 // look-back scan, place, stable within-cell rank + gather), density/pressure, force + integrate.
+// Every pass is one launch over all scenes of a context.
 //
 // Floating-point contract (shared with the CPU transcription): binary32, correctly rounded '/'
 // and sqrtf (-fhip-fp32-correctly-rounded-divide-sqrt), subnormals kept, no FMA contraction
@@ -46,12 +47,35 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
     return x * q + min(x, r) + k;
 }
 
-// Walks the in-grid neighbour cells of cell `key` in NEIGHBORHOOD order (density.comp:40-50:
-// dx outermost, dz innermost); visit(start, end) receives each cell's slot range and
-// end_plane() runs after every dx plane.  The cell loop is wave-uniform.
+// Scene owning logical block b: the largest s with first[s] <= b.  `first` has num_scenes + 1
+// entries; scenes without particles have first[s] == first[s+1] and are skipped.
+__device__ __forceinline__ uint32_t find_scene(const uint32_t* __restrict__ first, uint32_t num_scenes, uint32_t b) {
+    uint32_t lo = 0, hi = num_scenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ const uint32_t* first_blocks(const Scenes& S) {
+    return BLOCK == 64 ? S.blk64 : (BLOCK == 128 ? S.blk128 : S.blk256);
+}
+template <int BLOCK>
+__device__ __forceinline__ uint32_t total_blocks(const Scenes& S) {
+    return BLOCK == 64 ? S.blocks64 : (BLOCK == 128 ? S.blocks128 : S.blocks256);
+}
+
+// Walks the in-grid neighbour cells of scene-local cell `key` in NEIGHBORHOOD order
+// (density.comp:40-50: dx outermost, dz innermost); visit(start, end) receives each cell's slot
+// range (clipped to `limit`) and end_plane() runs after every dx plane.  The cell loop is
+// wave-uniform.
 template <typename Visit, typename EndPlane>
 __device__ __forceinline__ void for_each_neighbour_cell(uint32_t key, const uint2* __restrict__ range,
-                                                        const StepConsts& c, Visit& visit, EndPlane& end_plane) {
+                                                        const StepConsts& c, uint32_t limit, Visit& visit,
+                                                        EndPlane& end_plane) {
     const int R = c.R;
     const int cx = (int)(key % (uint32_t)R);
     const int cy = (int)((key / (uint32_t)R) % (uint32_t)R);
@@ -65,7 +89,7 @@ __device__ __forceinline__ void for_each_neighbour_cell(uint32_t key, const uint
             const int nz = cz + (dyz % 3 - 1);
             if (nx < 0 || ny < 0 || nz < 0 || nx >= R || ny >= R || nz >= R) continue;
             const uint2 rg = range[(nz * R + ny) * R + nx];
-            visit(rg.x, min(rg.x + rg.y, c.n));
+            visit(rg.x, min(rg.x + rg.y, limit));
         }
         end_plane();
     }
@@ -74,22 +98,39 @@ __device__ __forceinline__ void for_each_neighbour_cell(uint32_t key, const uint
 // ------------------------------------------------------------------------------------------
 // 1. cell hash + histogram (count.comp:25-36).  Consecutive lanes that fall in the same cell
 //    form one run; the run's first lane does a single atomicAdd for the whole run, every lane
-//    keeps its arrival rank.  Also resets the scan's look-back state for this step.
+//    keeps its arrival rank.  Also resets the scan's look-back state and the per-scene clocks
+//    for this step.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ posrho, uint32_t* __restrict__ key,
-                                                 uint32_t* __restrict__ rank, uint32_t* __restrict__ counts,
-                                                 unsigned long long* __restrict__ scan_status,
-                                                 uint32_t* __restrict__ scan_ticket, uint32_t num_tiles,
-                                                 StepConsts c) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < num_tiles) scan_status[i] = 0ull;
-    if (i == 0) *scan_ticket = 0u;
-
-    const bool valid = i < c.n;
-    uint32_t k = kNoKey;
-    if (valid) {
-        const float4 p = posrho[i];
-        k = cell_key(c, p.x, p.y, p.z);
+__global__ __launch_bounds__(kHashBlock) void k_hash(Scenes S, const float4* __restrict__ posrho,
+                                                     uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                     uint32_t* __restrict__ counts,
+                                                     unsigned long long* __restrict__ scan_status,
+                                                     uint32_t* __restrict__ scan_ticket,
+                                                     unsigned long long* __restrict__ scene_done,
+                                                     unsigned long long* __restrict__ step_start,
+                                                     uint32_t* __restrict__ rho_max) {
+    const uint32_t g = blockIdx.x * kHashBlock + threadIdx.x;
+    if (g < S.total_tiles) scan_status[g] = 0ull;
+    if (g < S.num_scenes) scene_done[g] = 0ull;
+    if (g == 0) {
+        *scan_ticket = 0u;
+        *step_start = wall_clock64();
+        *rho_max = 0u;
+    }
+    const uint32_t b = blockIdx.x;
+    bool valid = false;
+    uint32_t k = kNoKey, i = 0, c_off = 0;
+    if (b < S.blocks256) {
+        const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+        const SceneDesc& d = S.desc[s];
+        const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
+        valid = lt < d.c.n;
+        if (valid) {
+            i = d.p_off + lt;
+            c_off = d.c_off;
+            const float4 p = posrho[i];
+            k = cell_key(d.c, p.x, p.y, p.z);
+        }
     }
     const unsigned lane = lane_id();
     const uint32_t prev = __shfl_up(k, 1);
@@ -100,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ posr
     const unsigned long long above = leaders & ~upto;
     const int next = above ? (__ffsll((long long)above) - 1) : 64;
     uint32_t base = 0;
-    if (leader && valid) base = atomicAdd(&counts[k], (uint32_t)(next - first));
+    if (leader && valid) base = atomicAdd(&counts[c_off + k], (uint32_t)(next - first));
     base = __shfl(base, first);
     if (valid) {
         key[i] = k;
@@ -110,7 +151,8 @@ __global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ posr
 
 // ------------------------------------------------------------------------------------------
 // 2. exclusive scan of the per-cell counts (linearScan.comp:15-28), single pass with decoupled
-//    look-back.  Writes range[cell] = (offset, count) and zeroes counts for the next step.
+//    look-back, each scene on its own chain of tiles.  Writes range[cell] = (first slot, count)
+//    and zeroes the counts for the next step.
 //    Look-back word: (flag << 32) | value; flag 1 = tile aggregate, 2 = inclusive prefix.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long ld_status(const unsigned long long* p) {
@@ -125,9 +167,10 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ counts, uint2* __restrict__ range,
+__global__ __launch_bounds__(kScanThreads) void k_scan(Scenes S, uint32_t* __restrict__ counts,
+                                                       uint2* __restrict__ range,
                                                        unsigned long long* __restrict__ status,
-                                                       uint32_t* __restrict__ ticket, uint32_t ncells,
+                                                       uint32_t* __restrict__ ticket,
                                                        uint32_t* __restrict__ error_flag) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kScanThreads / 64];
@@ -137,12 +180,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ co
     const unsigned wave = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const size_t base = (size_t)tile * kScanTile + (size_t)tid * kScanItems;
+    const uint32_t tile = s_tile;  // tiles are handed out in order: earlier tiles are running
+    const uint32_t s = find_scene(S.tiles, S.num_scenes, tile);
+    const uint32_t first_tile = S.tiles[s];
+    const SceneDesc& d = S.desc[s];
+    const uint32_t ncells = d.c.ncells;
+    const size_t local = (size_t)(tile - first_tile) * kScanTile + (size_t)tid * kScanItems;
+    uint32_t* cnt = counts + d.c_off;
+    uint2* rng = range + d.c_off;
 
     uint32_t v[kScanItems];
-    if (base + kScanItems <= ncells) {
-        const uint4* src = reinterpret_cast<const uint4*>(counts + base);
+    if (local + kScanItems <= ncells) {
+        const uint4* src = reinterpret_cast<const uint4*>(cnt + local);
 #pragma unroll
         for (int q = 0; q < kScanItems / 4; ++q) {
             const uint4 w = src[q];
@@ -151,14 +200,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ co
             v[4 * q + 2] = w.z;
             v[4 * q + 3] = w.w;
         }
-        uint4* dst = reinterpret_cast<uint4*>(counts + base);
+        uint4* dst = reinterpret_cast<uint4*>(cnt + local);
 #pragma unroll
         for (int q = 0; q < kScanItems / 4; ++q) dst[q] = make_uint4(0u, 0u, 0u, 0u);
     } else {
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
-            v[j] = (base + j < ncells) ? counts[base + j] : 0u;
-            if (base + j < ncells) counts[base + j] = 0u;
+            v[j] = (local + j < ncells) ? cnt[local + j] : 0u;
+            if (local + j < ncells) cnt[local + j] = 0u;
         }
     }
     uint32_t sum = 0;
@@ -167,9 +216,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ co
 
     uint32_t incl = sum;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(incl, d);
-        if ((int)lane >= d) incl += t;
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const uint32_t t = __shfl_up(incl, dd);
+        if ((int)lane >= dd) incl += t;
     }
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
@@ -184,16 +233,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ co
 
     if (wave == 0) {
         uint32_t tile_prefix = 0;
-        if (tile == 0) {
-            if (lane == 0) st_status(&status[0], (2ull << 32) | tile_total);
+        if (tile == first_tile) {
+            if (lane == 0) st_status(&status[tile], (2ull << 32) | tile_total);
         } else {
             if (lane == 0) st_status(&status[tile], (1ull << 32) | tile_total);
             long long look = (long long)tile - 1;
             uint32_t acc = 0, spins = 0;
             for (;;) {
                 const long long idx = look - (long long)lane;
-                unsigned long long s = (idx >= 0) ? ld_status(&status[idx]) : (2ull << 32);
-                const uint32_t flag = (uint32_t)(s >> 32);
+                // tiles before the scene's first tile count as a zero prefix
+                unsigned long long w = (idx >= (long long)first_tile) ? ld_status(&status[idx]) : (2ull << 32);
+                const uint32_t flag = (uint32_t)(w >> 32);
                 const unsigned long long not_ready = __ballot(flag == 0u);
                 const unsigned long long is_prefix = __ballot(flag == 2u);
                 const int first_prefix = is_prefix ? (__ffsll((long long)is_prefix) - 1) : 64;
@@ -207,7 +257,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ co
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                const uint32_t contrib = ((int)lane <= first_prefix) ? (uint32_t)s : 0u;
+                const uint32_t contrib = ((int)lane <= first_prefix) ? (uint32_t)w : 0u;
                 acc += wave_sum(contrib);
                 if (first_prefix < 64) break;
                 look -= 64;
@@ -218,9 +268,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ co
         if (lane == 0) s_prefix = tile_prefix;
     }
     __syncthreads();
-    uint32_t run = s_prefix + thread_excl;
-    if (base + kScanItems <= ncells) {
-        uint4* dst = reinterpret_cast<uint4*>(range + base);
+    uint32_t run = d.p_off + s_prefix + thread_excl;
+    if (local + kScanItems <= ncells) {
+        uint4* dst = reinterpret_cast<uint4*>(rng + local);
 #pragma unroll
         for (int q = 0; q < kScanItems / 2; ++q) {
             const uint32_t a = run;
@@ -232,70 +282,83 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ co
     } else {
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
-            if (base + j < ncells) range[base + j] = make_uint2(run, v[j]);
+            if (local + j < ncells) rng[local + j] = make_uint2(run, v[j]);
             run += v[j];
         }
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// 3. place: slot = offset[cell] + arrival rank (reorder.comp:38-43 without the record copy).
+// 3. place: slot = first slot of the cell + arrival rank (reorder.comp:38-43 without the copy).
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
-                                                  const uint2* __restrict__ range, uint32_t* __restrict__ slot_tmp,
-                                                  uint32_t n) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = key[i];
-    const uint32_t slot = range[k].x + rank[i];
-    if (slot < n) slot_tmp[slot] = i;  // always true when the scan succeeded
+__global__ __launch_bounds__(kHashBlock) void k_place(Scenes S, const uint32_t* __restrict__ key,
+                                                      const uint32_t* __restrict__ rank,
+                                                      const uint2* __restrict__ range,
+                                                      uint32_t* __restrict__ slot_tmp) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+    const SceneDesc& d = S.desc[s];
+    const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
+    if (lt >= d.c.n) return;
+    const uint32_t i = d.p_off + lt;
+    const uint32_t slot = range[d.c_off + key[i]].x + rank[i];
+    if (slot - d.p_off < d.c.n) slot_tmp[slot] = i;  // always true when the scan succeeded
 }
 
 // ------------------------------------------------------------------------------------------
-// 4. stable rank + gather: the particle that arrived at slot t moves to offset + (number of
-//    particles of its cell with a smaller input index), so every cell keeps input order; the
-//    record is copied there (reorder.comp:44) and the cell index kept beside it.
+// 4. stable rank + gather: the particle that arrived at slot t moves to the cell's first slot +
+//    (number of particles of its cell with a smaller input index), so every cell keeps input
+//    order; the record is copied there (reorder.comp:44) and the cell index kept beside it.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ slot_tmp,
-                                                   const uint32_t* __restrict__ key, const uint2* __restrict__ range,
-                                                   const float4* __restrict__ posrho_in,
-                                                   const float4* __restrict__ velp_in, float4* __restrict__ posrho_out,
-                                                   float4* __restrict__ velp_out, uint32_t* __restrict__ sorted,
-                                                   uint32_t* __restrict__ skey, uint32_t n) {
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (t >= n) return;
+__global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t* __restrict__ slot_tmp,
+                                                       const uint32_t* __restrict__ key,
+                                                       const uint2* __restrict__ range,
+                                                       const float4* __restrict__ posrho_in,
+                                                       const float4* __restrict__ velp_in,
+                                                       float4* __restrict__ posrho_out, float4* __restrict__ velp_out,
+                                                       uint32_t* __restrict__ sorted, uint32_t* __restrict__ skey,
+                                                       const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id_out) {
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+    const SceneDesc& d = S.desc[s];
+    const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
+    if (lt >= d.c.n) return;
+    const uint32_t t = d.p_off + lt;
+    const uint32_t lim = d.p_off + d.c.n;
     const uint32_t i = slot_tmp[t];
-    if (i >= n) return;  // only after a failed scan (error flag set)
+    if (i - d.p_off >= d.c.n) return;  // only after a failed scan (error flag set)
     const uint32_t k = key[i];
-    const uint2 rg = range[k];
-    const uint32_t end = min(rg.x + rg.y, n);
+    const uint2 rg = range[d.c_off + k];
+    const uint32_t end = min(rg.x + rg.y, lim);
     uint32_t r = 0;
     for (uint32_t u = rg.x; u < end; ++u) r += (slot_tmp[u] < i) ? 1u : 0u;
     const uint32_t dst = rg.x + r;
-    if (dst >= n) return;
+    if (dst >= lim) return;
     sorted[dst] = i;
     skey[dst] = k;
     posrho_out[dst] = posrho_in[i];
     velp_out[dst] = velp_in[i];
+    if (id_out) id_out[dst] = id_in[i];
 }
 
 // ------------------------------------------------------------------------------------------
 // Neighbour search shared by the density and the force pass.
 //
 // Each lane owns one particle.  The 27 neighbour cells are visited in NEIGHBORHOOD order with a
-// wave-uniform cell loop (for_each_neighbour_cell); inside a cell each lane walks its own slot
-// range CHUNK slots at a time (CHUNK loads in flight).  Candidate tests are branch-free: every
-// candidate is written to the lane's next entry of an LDS list and the entry count advances by
-// the test's outcome.  The list is drained after every dx plane, or when the next chunk could
-// overflow it, or only when full and at the end (PLANE_DRAIN false): the per-neighbour arithmetic then runs over compacted entries with all lanes
-// busy instead of under a mask where ~15% of the lanes accept a given candidate.  Entries are
-// drained in enumeration order, so every sum is formed exactly as the reference forms it, and
-// the result does not depend on BLOCK, CAP, CHUNK or PLANE_DRAIN (the tuning knobs of the passes).
+// wave-uniform cell loop; inside a cell each lane walks its own slot range CHUNK slots at a time
+// (CHUNK loads in flight).  Candidate tests are branch-free: every candidate is written to the
+// lane's next entry of an LDS list and the entry count advances by the test's outcome.  The list
+// is drained when the next chunk could overflow it, at the end, and (PLANE_DRAIN) after every dx
+// plane: the per-neighbour arithmetic then runs over compacted entries with all lanes busy
+// instead of under a mask where ~15% of the lanes accept a given candidate.  Entries are drained
+// in enumeration order, so every sum is formed exactly as the reference forms it, and the
+// result does not depend on BLOCK, CAP, CHUNK or PLANE_DRAIN (the tuning knobs of the passes).
 // ------------------------------------------------------------------------------------------
 template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, typename Entry, typename Drain>
-__device__ __forceinline__ void search_neighbours(uint32_t key, uint32_t t, float px, float py, float pz,
-                                                  const float4* __restrict__ pos, const uint2* __restrict__ range,
-                                                  const StepConsts& c, Entry* __restrict__ list, Drain& drain_fn) {
+__device__ __forceinline__ void search_neighbours(uint32_t key, uint32_t t, uint32_t limit, float px, float py,
+                                                  float pz, const float4* __restrict__ pos,
+                                                  const uint2* __restrict__ range, const StepConsts& c,
+                                                  Entry* __restrict__ list, Drain& drain_fn) {
     static_assert(CAP >= CHUNK, "list must hold a chunk");
     int cnt = 0;
     auto drain = [&]() {
@@ -327,7 +390,7 @@ __device__ __forceinline__ void search_neighbours(uint32_t key, uint32_t t, floa
     auto end_plane = [&]() {
         if (PLANE_DRAIN) drain();
     };
-    for_each_neighbour_cell(key, range, c, visit, end_plane);
+    for_each_neighbour_cell(key, range, c, limit, visit, end_plane);
     drain();
 }
 
@@ -354,22 +417,33 @@ __device__ __forceinline__ float eos(const StepConsts& c, float rho) {
 }
 
 template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN>
-__global__ __launch_bounds__(BLOCK) void k_density(float4* __restrict__ posrho, float4* __restrict__ velp,
+__global__ __launch_bounds__(BLOCK) void k_density(Scenes S, float4* __restrict__ posrho, float4* __restrict__ velp,
                                                    const uint32_t* __restrict__ skey,
-                                                   const uint2* __restrict__ range, StepConsts c) {
+                                                   const uint2* __restrict__ range, uint32_t* __restrict__ rho_max) {
     __shared__ float s_list[CAP * BLOCK];
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (t >= c.n) return;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
+    const SceneDesc& d = S.desc[s];
+    const StepConsts& c = d.c;
+    const uint32_t lt = (b - first_blocks<BLOCK>(S)[s]) * BLOCK + threadIdx.x;
+    if (lt >= c.n) return;
+    const uint32_t t = d.p_off + lt;
     const float4 p = posrho[t];
     float* list = s_list + threadIdx.x;
     float rho = c.m * poly6(c, 0.0f);
     auto drain = [&](int cnt) {
         for (int k = 0; k < cnt; ++k) rho += c.m * poly6(c, sqrtf(list[k * BLOCK]));
     };
-    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, p.x, p.y, p.z, posrho, range, c, list, drain);
+    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho,
+                                                      range + d.c_off, c, list, drain);
     const float wall = c.wall_density ? wall_density(c, p.x, p.y, p.z) : 0.0f;
-    reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = rho + wall;
+    const float density = rho + wall;
+    reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = density;
     reinterpret_cast<float*>(velp)[4 * (size_t)t + 3] = eos(c, rho);
+    // largest density of the pass (positive floats order as their bit patterns; NaN ranks highest)
+    uint32_t bits = __float_as_uint(density) & 0x7fffffffu;
+    for (int d = 32; d >= 1; d >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, d));
+    if ((threadIdx.x & 63) == 0) atomicMax(rho_max, bits);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -477,15 +551,67 @@ struct ForceSums {
     }
 };
 
-template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN>
-__global__ __launch_bounds__(BLOCK) void k_update(const float4* __restrict__ posrho_in,
+// Integration of the learned large step: the reference's update with the network's correction
+// added to the acceleration (far from walls) and the plain acceleration optionally written out.
+__device__ __forceinline__ void learned_axis(const StepConsts& c, float pf, float vf, float ext, float denom,
+                                             float v0, float x0, float dt, float corr, float& acc_plain, float& vel,
+                                             float& pos) {
+    vf = vf * c.mu;
+    const float force = (pf + vf) + ext;
+    acc_plain = force / denom;
+    const float v = v0 + (acc_plain + corr) * dt;
+    vel = fminf(fmaxf(v, -c.max_speed), c.max_speed);
+    pos = x0 + vel * dt;
+    if (pos < c.border) {
+        vel = vel * -c.damping;
+        pos = c.border;
+    } else if (pos > c.hi) {
+        vel = vel * -c.damping;
+        pos = c.hi;
+    }
+}
+
+__device__ __forceinline__ void learned_integrate(const StepConsts& c, const ForceSums& f, const V3& ext, float denom,
+                                                  const float4& p, const float4& pv, uint32_t t, float dt,
+                                                  const float4* __restrict__ corr, float corr_scale,
+                                                  float4* __restrict__ acc_out, float4& po, float4& vo) {
+    float4 k = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float margin = 2.0f * c.h;
+    const bool interior = p.x > margin && p.y > margin && p.z > margin && p.x < c.size - margin &&
+                          p.y < c.size - margin && p.z < c.size - margin;
+    if (corr && interior) {
+        const float4 q = corr[t];
+        k = make_float4(q.x * corr_scale, q.y * corr_scale, q.z * corr_scale, 0.0f);
+    }
+    float4 a;
+    learned_axis(c, f.pfx, f.vfx, ext.x, denom, pv.x, p.x, dt, k.x, a.x, vo.x, po.x);
+    learned_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, k.y, a.y, vo.y, po.y);
+    learned_axis(c, f.pfz, f.vfz, ext.z, denom, pv.z, p.z, dt, k.z, a.z, vo.z, po.z);
+    a.w = interior ? 1.0f : 0.0f;
+    if (acc_out) acc_out[t] = a;
+}
+
+// MODE 0: the reference's force pass.  MODE 1 (learned closure): dt comes from the argument and
+// corr * corr_scale is added to the acceleration of particles at least 2h from every wall; with
+// acc_out the plain acceleration is also written out.
+template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __restrict__ posrho_in,
                                                   const float4* __restrict__ velp_in,
                                                   const uint32_t* __restrict__ skey,
                                                   const uint2* __restrict__ range, float4* __restrict__ posrho_out,
-                                                  float4* __restrict__ velp_out, StepConsts c, float dt) {
+                                                  float4* __restrict__ velp_out,
+                                                  unsigned long long* __restrict__ scene_done,
+                                                  const uint32_t* __restrict__ id_sorted, uint32_t* __restrict__ id_out,
+                                                  float dt_arg, const float4* __restrict__ corr, float corr_scale,
+                                                  float4* __restrict__ acc_out) {
     __shared__ uint32_t s_list[CAP * BLOCK];
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (t >= c.n) return;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
+    const SceneDesc& d = S.desc[s];
+    const StepConsts& c = d.c;
+    const uint32_t lt = (b - first_blocks<BLOCK>(S)[s]) * BLOCK + threadIdx.x;
+    if (lt >= c.n) return;
+    const uint32_t t = d.p_off + lt;
     const float4 p = posrho_in[t];
     const float4 pv = velp_in[t];
     uint32_t* list = s_list + threadIdx.x;
@@ -496,7 +622,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(const float4* __restrict__ pos
             f.add(c, p, pv, posrho_in[u], velp_in[u]);
         }
     };
-    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, p.x, p.y, p.z, posrho_in, range, c, list, drain);
+    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho_in,
+                                                      range + d.c_off, c, list, drain);
 
     V3 ext{c.g[0] * p.w, c.g[1] * p.w, c.g[2] * p.w};
     V3 mouse{0.0f, 0.0f, 0.0f}, wall{0.0f, 0.0f, 0.0f};
@@ -508,71 +635,172 @@ __global__ __launch_bounds__(BLOCK) void k_update(const float4* __restrict__ pos
 
     const float denom = p.w + 1e-16f;
     float4 po, vo;
-    integrate_axis(c, f.pfx, f.vfx, ext.x, denom, pv.x, p.x, dt, vo.x, po.x);
-    integrate_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, vo.y, po.y);
-    integrate_axis(c, f.pfz, f.vfz, ext.z, denom, pv.z, p.z, dt, vo.z, po.z);
+    if (MODE == 0) {
+        const float dt = d.dt;
+        integrate_axis(c, f.pfx, f.vfx, ext.x, denom, pv.x, p.x, dt, vo.x, po.x);
+        integrate_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, vo.y, po.y);
+        integrate_axis(c, f.pfz, f.vfz, ext.z, denom, pv.z, p.z, dt, vo.z, po.z);
+    } else {
+        learned_integrate(c, f, ext, denom, p, pv, t, dt_arg, corr, corr_scale, acc_out, po, vo);
+    }
     po.w = p.w;
     vo.w = pv.w;
     posrho_out[t] = po;
     velp_out[t] = vo;
+    if (id_out) id_out[t] = id_sorted[t];
+    // completion time of this wave's particles; the scene's latest wave ends its step
+    if ((threadIdx.x & 63) == 0) atomicMax(&scene_done[s], (unsigned long long)wall_clock64());
+}
+
+
+// ------------------------------------------------------------------------------------------
+// Learned closure features (single-scene contexts).  For every requested slot of the sorted
+// buffer: the 32 nearest neighbours inside the kernel radius (the density pass's acceptance
+// test; ascending squared distance, ties in NEIGHBORHOOD enumeration order), each as
+// ((x_j - x_i) / h, (v_j - v_i) / v_ref), then rho_i / rho0 - 1 scaled by 10, the step
+// multiplier / 4 and the fraction of the 32 places filled; zero padded to 208 bf16 values.
+// ------------------------------------------------------------------------------------------
+constexpr int kFeatBlock = 64;
+
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+
+__global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const float4* __restrict__ posrho,
+                                                                 const float4* __restrict__ velp,
+                                                                 const uint32_t* __restrict__ skey,
+                                                                 const uint2* __restrict__ range,
+                                                                 const uint32_t* __restrict__ rows, uint32_t nrows,
+                                                                 float dt_mult, float inv_v, uint16_t* __restrict__ X) {
+    __shared__ float s_d2[kClosureNeighbours * kFeatBlock];
+    __shared__ uint32_t s_slot[kClosureNeighbours * kFeatBlock];
+    const uint32_t r = blockIdx.x * kFeatBlock + threadIdx.x;
+    if (r >= nrows) return;
+    const SceneDesc& d = S.desc[0];
+    const StepConsts& c = d.c;
+    const uint32_t t = rows ? rows[r] : d.p_off + r;
+    uint4* row = reinterpret_cast<uint4*>(X + (size_t)r * kClosureFeatures);
+    if (t - d.p_off >= c.n) {
+        for (int q = 0; q < kClosureFeatures / 8; ++q) row[q] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const float4 p = posrho[t];
+    const float4 pv = velp[t];
+    float* l_d2 = s_d2 + threadIdx.x;
+    uint32_t* l_slot = s_slot + threadIdx.x;
+    int cnt = 0;
+    auto visit = [&](uint32_t start, uint32_t end) {
+        for (uint32_t u = start; u < end; ++u) {
+            if (u == t) continue;
+            const float4 q = posrho[u];
+            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+            const float d2 = (rx * rx + ry * ry) + rz * rz;
+            if (d2 >= c.cut2) continue;
+            if (cnt == kClosureNeighbours && !(d2 < l_d2[(kClosureNeighbours - 1) * kFeatBlock])) continue;
+            int pos = cnt < kClosureNeighbours ? cnt : kClosureNeighbours - 1;
+            while (pos > 0 && l_d2[(pos - 1) * kFeatBlock] > d2) {
+                l_d2[pos * kFeatBlock] = l_d2[(pos - 1) * kFeatBlock];
+                l_slot[pos * kFeatBlock] = l_slot[(pos - 1) * kFeatBlock];
+                --pos;
+            }
+            l_d2[pos * kFeatBlock] = d2;
+            l_slot[pos * kFeatBlock] = u;
+            if (cnt < kClosureNeighbours) ++cnt;
+        }
+    };
+    auto nothing = []() {};
+    for_each_neighbour_cell(skey[t], range + d.c_off, c, d.p_off + c.n, visit, nothing);
+
+    const float inv_h = 1.0f / c.h;
+    // eight groups of four neighbours: 24 values = three 16-byte stores
+    for (int g = 0; g < kClosureNeighbours / 4; ++g) {
+        float v[24];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = g * 4 + j;
+            if (k < cnt) {
+                const uint32_t u = l_slot[k * kFeatBlock];
+                const float4 q = posrho[u];
+                const float4 qv = velp[u];
+                v[6 * j + 0] = (q.x - p.x) * inv_h;
+                v[6 * j + 1] = (q.y - p.y) * inv_h;
+                v[6 * j + 2] = (q.z - p.z) * inv_h;
+                v[6 * j + 3] = (qv.x - pv.x) * inv_v;
+                v[6 * j + 4] = (qv.y - pv.y) * inv_v;
+                v[6 * j + 5] = (qv.z - pv.z) * inv_v;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 6; ++e) v[6 * j + e] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            row[g * 3 + q] = make_uint4(pack_bf16(v[8 * q + 0], v[8 * q + 1]), pack_bf16(v[8 * q + 2], v[8 * q + 3]),
+                                        pack_bf16(v[8 * q + 4], v[8 * q + 5]), pack_bf16(v[8 * q + 6], v[8 * q + 7]));
+    }
+    const float rho_rel = (p.w / c.rho0 - 1.0f) * 10.0f;
+    row[24] = make_uint4(pack_bf16(rho_rel, dt_mult * 0.25f), pack_bf16((float)cnt * (1.0f / kClosureNeighbours), 0.0f),
+                         0u, 0u);
+    row[25] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // ------------------------------------------------------------------------------------------
-// Record <-> split-half conversion and fills.
+// Record <-> split-half conversion.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_split(const float4* __restrict__ rec, float4* __restrict__ posrho,
-                                                  float4* __restrict__ velp, uint32_t n) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kHashBlock) void k_split(const float4* __restrict__ rec, float4* __restrict__ posrho,
+                                                      float4* __restrict__ velp, uint32_t n) {
+    const uint32_t i = blockIdx.x * kHashBlock + threadIdx.x;
     if (i >= n) return;
     posrho[i] = rec[2 * (size_t)i];
     velp[i] = rec[2 * (size_t)i + 1];
 }
 
-__global__ __launch_bounds__(kBlock) void k_join(const float4* __restrict__ posrho,
-                                                 const float4* __restrict__ velp, float4* __restrict__ rec,
-                                                 uint32_t n) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kHashBlock) void k_join(const float4* __restrict__ posrho,
+                                                     const float4* __restrict__ velp, float4* __restrict__ rec,
+                                                     uint32_t n) {
+    const uint32_t i = blockIdx.x * kHashBlock + threadIdx.x;
     if (i >= n) return;
     rec[2 * (size_t)i] = posrho[i];
     rec[2 * (size_t)i + 1] = velp[i];
-}
-
-__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, size_t n) {
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) p[i] = value;
 }
 
 inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
 
-hipError_t launch_hash(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
-    const size_t work = c.n > b.num_scan_tiles ? c.n : b.num_scan_tiles;
-    if (work == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hash, dim3(blocks_for(work, kBlock)), dim3(kBlock), 0, s, b.posrho[0], b.key, b.rank,
-                       b.counts, b.scan_status, b.scan_ticket, b.num_scan_tiles, c);
+hipError_t launch_hash(const DeviceBuffers& b, hipStream_t s) {
+    const Scenes& S = b.sc;
+    size_t blocks = S.blocks256;
+    const size_t reset = blocks_for(S.total_tiles > S.num_scenes ? S.total_tiles : S.num_scenes, kHashBlock);
+    if (reset > blocks) blocks = reset;
+    if (blocks == 0) blocks = 1;  // still resets the ticket and the step clock
+    hipLaunchKernelGGL(k_hash, dim3((unsigned)blocks), dim3(kHashBlock), 0, s, S, b.posrho[0], b.key, b.rank,
+                       b.counts, b.scan_status, b.scan_ticket, b.scene_done, b.step_start, b.rho_max);
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
-    const unsigned tiles = blocks_for(c.ncells, kScanTile);
-    if (tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(kScanThreads), 0, s, b.counts, b.range, b.scan_status,
-                       b.scan_ticket, c.ncells, b.error_flag);
+hipError_t launch_scan(const DeviceBuffers& b, hipStream_t s) {
+    if (b.sc.total_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan, dim3(b.sc.total_tiles), dim3(kScanThreads), 0, s, b.sc, b.counts, b.range,
+                       b.scan_status, b.scan_ticket, b.error_flag);
     return hipGetLastError();
 }
 
-hipError_t launch_place(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
-    if (c.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_place, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.key, b.rank, b.range,
-                       b.slot_tmp, c.n);
+hipError_t launch_place(const DeviceBuffers& b, hipStream_t s) {
+    if (b.sc.blocks256 == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_place, dim3(b.sc.blocks256), dim3(kHashBlock), 0, s, b.sc, b.key, b.rank, b.range,
+                       b.slot_tmp);
     return hipGetLastError();
 }
 
-hipError_t launch_gather(const DeviceBuffers& b, const StepConsts& c, hipStream_t s) {
-    if (c.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, b.slot_tmp, b.key, b.range,
-                       b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, c.n);
+hipError_t launch_gather(const DeviceBuffers& b, hipStream_t s) {
+    if (b.sc.blocks256 == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(b.sc.blocks256), dim3(kHashBlock), 0, s, b.sc, b.slot_tmp, b.key, b.range,
+                       b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, b.ids[0], b.ids[1]);
     return hipGetLastError();
 }
 
@@ -599,37 +827,61 @@ static const PassVariant kVariants[] = {
 int num_variants() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
 PassVariant variant(int v) { return kVariants[(v >= 0 && v < num_variants()) ? v : 0]; }
 
-hipError_t launch_density(const DeviceBuffers& b, const StepConsts& c, int v, hipStream_t s) {
-    if (c.n == 0) return hipSuccess;
+static unsigned pass_blocks(const Scenes& S, int block) {
+    return block == 64 ? S.blocks64 : (block == 128 ? S.blocks128 : S.blocks256);
+}
+
+hipError_t launch_density(const DeviceBuffers& b, int v, hipStream_t s) {
     const PassVariant pv = variant(v);
-#define WC_LAUNCH(B, C, K, P)                                                                                 \
-    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                         \
-        hipLaunchKernelGGL((k_density<B, C, K, P>), dim3(blocks_for(c.n, B)), dim3(B), 0, s, b.posrho[1], b.velp[1], \
-                           b.skey, b.range, c);                                                              \
-        return hipGetLastError();                                                                            \
+    const unsigned blocks = pass_blocks(b.sc, pv.block);
+    if (blocks == 0) return hipSuccess;
+#define WC_LAUNCH(B, C, K, P)                                                                        \
+    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                \
+        hipLaunchKernelGGL((k_density<B, C, K, P>), dim3(blocks), dim3(B), 0, s, b.sc, b.posrho[1], \
+                           b.velp[1], b.skey, b.range, b.rho_max);                                  \
+        return hipGetLastError();                                                                   \
     }
     WC_VARIANTS(WC_LAUNCH)
 #undef WC_LAUNCH
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_update(const DeviceBuffers& b, const StepConsts& c, float dt, int v, hipStream_t s) {
-    if (c.n == 0) return hipSuccess;
+hipError_t launch_update(const DeviceBuffers& b, int v, hipStream_t s) {
     const PassVariant pv = variant(v);
-#define WC_LAUNCH(B, C, K, P)                                                                                 \
-    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                         \
-        hipLaunchKernelGGL((k_update<B, C, K, P>), dim3(blocks_for(c.n, B)), dim3(B), 0, s, b.posrho[1], b.velp[1], \
-                           b.skey, b.range, b.posrho[0], b.velp[0], c, dt);                                  \
-        return hipGetLastError();                                                                            \
+    const unsigned blocks = pass_blocks(b.sc, pv.block);
+    if (blocks == 0) return hipSuccess;
+#define WC_LAUNCH(B, C, K, P)                                                                        \
+    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                \
+        hipLaunchKernelGGL((k_update<B, C, K, P, 0>), dim3(blocks), dim3(B), 0, s, b.sc, b.posrho[1],\
+                           b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],\
+                           b.ids[0], 0.0f, nullptr, 0.0f, nullptr);                                 \
+        return hipGetLastError();                                                                   \
     }
     WC_VARIANTS(WC_LAUNCH)
 #undef WC_LAUNCH
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows, uint32_t nrows, float dt_mult,
+                                   float v_ref, uint16_t* X, hipStream_t s) {
+    if (nrows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(nrows, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc,
+                       b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, 1.0f / v_ref, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
+                                 float* acc_out, hipStream_t s) {
+    if (b.sc.blocks64 == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_update<64, 32, 4, true, 1>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc, b.posrho[1],
+                       b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1], b.ids[0], dt,
+                       reinterpret_cast<const float4*>(corr), corr_scale, reinterpret_cast<float4*>(acc_out));
+    return hipGetLastError();
 }
 
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_split, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_split, dim3(blocks_for(n, kHashBlock)), dim3(kHashBlock), 0, s,
                        reinterpret_cast<const float4*>(rec), posrho, velp, n);
     return hipGetLastError();
 }
@@ -637,14 +889,8 @@ hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* 
 hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,
                                hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_join, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, posrho, velp,
+    hipLaunchKernelGGL(k_join, dim3(blocks_for(n, kHashBlock)), dim3(kHashBlock), 0, s, posrho, velp,
                        reinterpret_cast<float4*>(rec), n);
-    return hipGetLastError();
-}
-
-hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p, value, n);
     return hipGetLastError();
 }
 

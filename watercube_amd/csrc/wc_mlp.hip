@@ -1,0 +1,418 @@
+// wc_mlp.hip -- the learned closure's network on gfx950 matrix cores.
+//
+// MLP 195 -> 128 -> 128 -> 128 -> 3 with SiLU, bf16 weights and activations, fp32 accumulation,
+// all products on v_mfma_f32_32x32x16_bf16.  The input is padded to 208 features and the output
+// to 32 columns (zero weights), so every product is a whole number of 32x32x16 MFMA tiles.
+//
+// Training (forward with stored activations, backward, Adam) is a sequence of one generic MFMA
+// GEMM kernel with strided operands and a few elementwise kernels.  Inference is one fused
+// kernel: a workgroup of four waves takes 128 particles, each wave 32 of them through all four
+// layers, activations staying in LDS between layers.
+//
+// MFMA 32x32x16 bf16 lane layout (wave64, lane l): A operand = A[l % 32][8 (l / 32) + 0..7],
+// B operand = B[8 (l / 32) + 0..7][l % 32], accumulator element r = C[8 (r / 4) + 4 (l / 32) +
+// r % 4][l % 32].
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "wc_mlp.hpp"
+
+namespace wc {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    // round to nearest even; NaN stays NaN
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+__device__ __forceinline__ f32x16 mfma(uint4 a, uint4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+// ------------------------------------------------------------------------------------------
+// Generic GEMM: C[m][n] (+)= sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) =
+// B[k*sbk + n*sbn] (bf16), C fp32 row-major with leading dimension ldc.  64x64 tile per
+// workgroup (2x2 waves of 32x32), K in steps of 16, split over gridDim.z with atomic adds.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, long sam, long sak,
+                                              const uint16_t* __restrict__ B, long sbk, long sbn,
+                                              float* __restrict__ C, long ldc, int M, int N, int K, int k_split,
+                                              int atomic) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[64 * 16];  // [m][k]
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[64 * 16];  // [n][k]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int kbeg = blockIdx.z * k_split;
+    const int kend = min(K, kbeg + k_split);
+    f32x16 acc = {};
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid * 4 + q;
+            const int r = e >> 4, kk = e & 15;
+            const int gk = k0 + kk;
+            const int gm = m0 + r, gn = n0 + r;
+            As[e] = (gm < M && gk < kend) ? A[(long)gm * sam + (long)gk * sak] : (uint16_t)0;
+            Bs[e] = (gn < N && gk < kend) ? B[(long)gk * sbk + (long)gn * sbn] : (uint16_t)0;
+        }
+        __syncthreads();
+        const int kq = (lane >> 5) * 8;
+        const uint4 a = *reinterpret_cast<const uint4*>(&As[(wm * 32 + (lane & 31)) * 16 + kq]);
+        const uint4 b = *reinterpret_cast<const uint4*>(&Bs[(wn * 32 + (lane & 31)) * 16 + kq]);
+        acc = mfma(a, b, acc);
+        __syncthreads();
+    }
+    const int col = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (row < M && col < N) {
+            float* dst = &C[(long)row * ldc + col];
+            if (atomic) atomicAdd(dst, acc[r]);
+            else *dst = acc[r];
+        }
+    }
+}
+
+// z += bias (kept in Z), H = bf16(silu(z)); rows of Z and H have `ld` columns, n of them used.
+__global__ void k_bias_silu(float* __restrict__ Z, const float* __restrict__ bias, uint16_t* __restrict__ H, int M,
+                            int n, int ld) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)M * ld) return;
+    const int c = (int)(i % ld);
+    if (c >= n) {
+        H[i] = 0;
+        return;
+    }
+    const float z = Z[i] + bias[c];
+    Z[i] = z;
+    H[i] = f2bf(silu(z));
+}
+
+// Output layer: y = acc + b4 (first kOut of kOutPad columns).  With targets: D4 = bf16(scale (y - t)),
+// loss += sum (y - t)^2.  Y receives y in 4-float rows.
+__global__ void k_output(const float* __restrict__ acc, const float* __restrict__ b4, const float* __restrict__ T,
+                         float* __restrict__ Y, uint16_t* __restrict__ D4, float scale, float* __restrict__ loss,
+                         int M) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (m < M) {
+        float y[kMlpOut];
+#pragma unroll
+        for (int c = 0; c < kMlpOut; ++c) y[c] = acc[(long)m * kMlpOutPad + c] + b4[c];
+        if (Y) {
+            float4 o = make_float4(y[0], y[1], y[2], 0.0f);
+            reinterpret_cast<float4*>(Y)[m] = o;
+        }
+        if (T) {
+#pragma unroll
+            for (int c = 0; c < kMlpOutPad; ++c) {
+                float d = 0.0f;
+                if (c < kMlpOut) {
+                    d = y[c] - T[(long)m * 4 + c];
+                    l += d * d;
+                }
+                D4[(long)m * kMlpOutPad + c] = f2bf(scale * d);
+            }
+        }
+    }
+    if (loss) {
+        for (int d = 32; d >= 1; d >>= 1) l += __shfl_xor(l, d);
+        if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+    }
+}
+
+// D = bf16(G * silu'(Z)), silu'(z) = s (1 + z (1 - s)), s = sigmoid(z)
+__global__ void k_silu_backward(const float* __restrict__ G, const float* __restrict__ Z, uint16_t* __restrict__ D,
+                                long count) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float z = Z[i];
+    const float s = 1.0f / (1.0f + __expf(-z));
+    D[i] = f2bf(G[i] * (s * (1.0f + z * (1.0f - s))));
+}
+
+// out[c] += sum over rows of D[m][c] (bias gradients), D bf16 with ld columns
+__global__ void k_colsum(const uint16_t* __restrict__ D, float* __restrict__ out, int M, int n, int ld) {
+    const int c = blockIdx.x;
+    float s = 0.0f;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) s += bf2f(D[(long)m * ld + c]);
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    __shared__ float part[16];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        if (c < n) out[c] += t;
+    }
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies of the master weights: row-major [k][n] for the GEMMs and transposed [n][k] for
+// the fused inference kernel.
+__global__ void k_cast(const float* __restrict__ w, uint16_t* __restrict__ rowmajor, uint16_t* __restrict__ transposed,
+                       int K, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K * N) return;
+    const int k = i / N, n = i % N;
+    const uint16_t h = f2bf(w[i]);
+    rowmajor[i] = h;
+    transposed[(long)n * K + k] = h;
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused inference: Y[n x 4] = MLP(X[n x 208]).  256 threads = 4 waves, 128 rows per workgroup.
+// ------------------------------------------------------------------------------------------
+struct InferWeights {
+    const uint16_t* w1t;  // [128][208]
+    const uint16_t* w2t;  // [128][128]
+    const uint16_t* w3t;  // [128][128]
+    const uint16_t* w4t;  // [32][128]
+    const float *b1, *b2, *b3, *b4;
+};
+
+template <int KDIM>
+__device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ act, int act_ld, bool act_global,
+                                             int act_rows_left, const uint16_t* __restrict__ wt,
+                                             const float* __restrict__ bias, uint16_t* __restrict__ out_lds, int lane) {
+    f32x16 acc[4] = {{}, {}, {}, {}};
+    const int row = lane & 31, kq = (lane >> 5) * 8;
+#pragma unroll 2
+    for (int k0 = 0; k0 < KDIM; k0 += 16) {
+        uint4 a = make_uint4(0u, 0u, 0u, 0u);
+        if (!act_global || row < act_rows_left) a = *reinterpret_cast<const uint4*>(&act[(long)row * act_ld + k0 + kq]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint4 b = *reinterpret_cast<const uint4*>(&wt[(long)(j * 32 + row) * KDIM + k0 + kq]);
+            acc[j] = mfma(a, b, acc[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = j * 32 + row;
+        const float bj = bias[col];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            out_lds[rr * kMlpHidden + col] = f2bf(silu(acc[j][r] + bj));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ X, InferWeights W,
+                                                   float* __restrict__ Y, uint32_t n) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_h[4][2][32 * kMlpHidden];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long row0 = ((long)blockIdx.x * 4 + wave) * 32;
+    // rows past n read zeros and write nothing; every wave reaches every barrier
+    const int rows_left = (int)max(0L, min((long)32, (long)n - row0));
+    uint16_t* h0 = s_h[wave][0];
+    uint16_t* h1 = s_h[wave][1];
+    hidden_layer<kMlpInPad>(X + (rows_left ? row0 : 0) * kMlpInPad, kMlpInPad, true, rows_left, W.w1t, W.b1, h0, lane);
+    __syncthreads();
+    hidden_layer<kMlpHidden>(h0, kMlpHidden, false, 32, W.w2t, W.b2, h1, lane);
+    __syncthreads();
+    hidden_layer<kMlpHidden>(h1, kMlpHidden, false, 32, W.w3t, W.b3, h0, lane);
+    __syncthreads();
+    // output layer: one 32x32 tile, columns 0..2 used
+    f32x16 acc = {};
+    const int row = lane & 31, kq = (lane >> 5) * 8;
+#pragma unroll
+    for (int k0 = 0; k0 < kMlpHidden; k0 += 16) {
+        const uint4 a = *reinterpret_cast<const uint4*>(&h0[row * kMlpHidden + k0 + kq]);
+        const uint4 b = *reinterpret_cast<const uint4*>(&W.w4t[(long)row * kMlpHidden + k0 + kq]);
+        acc = mfma(a, b, acc);
+    }
+    const int col = row;  // output column of this lane
+    if (col < 4) {
+        const float bc = col < kMlpOut ? W.b4[col] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (rr < rows_left) Y[(row0 + rr) * 4 + col] = col < kMlpOut ? acc[r] + bc : 0.0f;
+        }
+    }
+}
+
+inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------
+hipError_t gemm(const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C, long ldc, int M,
+                int N, int K, bool accumulate, hipStream_t s) {
+    if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+    // split K so that long reductions (weight gradients over a batch) spread over the chip
+    int splits = 1;
+    const long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
+    if (K >= 4096 && tiles < 256) splits = (int)std::min<long>((K + 2047) / 2048, 512 / tiles + 1);
+    int k_split = (K + splits - 1) / splits;
+    k_split = (k_split + 15) / 16 * 16;
+    splits = (K + k_split - 1) / k_split;
+    const bool atomic = accumulate || splits > 1;
+    if (atomic && !accumulate) {
+        hipError_t e = hipMemset2DAsync(C, ldc * sizeof(float), 0, N * sizeof(float), M, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_gemm, dim3((N + 63) / 64, (M + 63) / 64, splits), dim3(256), 0, s, A, sam, sak, B, sbk, sbn,
+                       C, ldc, M, N, K, k_split, atomic ? 1 : 0);
+    return hipGetLastError();
+}
+
+MlpLayout mlp_layout() {
+    MlpLayout L;
+    size_t o = 0;
+    L.w1 = o; o += (size_t)kMlpInPad * kMlpHidden;
+    L.b1 = o; o += kMlpHidden;
+    L.w2 = o; o += (size_t)kMlpHidden * kMlpHidden;
+    L.b2 = o; o += kMlpHidden;
+    L.w3 = o; o += (size_t)kMlpHidden * kMlpHidden;
+    L.b3 = o; o += kMlpHidden;
+    L.w4 = o; o += (size_t)kMlpHidden * kMlpOutPad;
+    L.b4 = o; o += kMlpOutPad;
+    L.total = o;
+    return L;
+}
+
+hipError_t mlp_refresh_bf16(Mlp& m, hipStream_t s) {
+    const MlpLayout L = mlp_layout();
+    struct {
+        size_t off;
+        int K, N;
+        uint16_t *rm, *tr;
+    } mats[4] = {{L.w1, kMlpInPad, kMlpHidden, m.w1, m.w1t},
+                 {L.w2, kMlpHidden, kMlpHidden, m.w2, m.w2t},
+                 {L.w3, kMlpHidden, kMlpHidden, m.w3, m.w3t},
+                 {L.w4, kMlpHidden, kMlpOutPad, m.w4, m.w4t}};
+    for (auto& x : mats) {
+        hipLaunchKernelGGL(k_cast, dim3(nblk((long)x.K * x.N, 256)), dim3(256), 0, s, m.master + x.off, x.rm, x.tr, x.K,
+                           x.N);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t mlp_infer(const Mlp& m, const uint16_t* X, float* Y, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const MlpLayout L = mlp_layout();
+    InferWeights W{m.w1t, m.w2t, m.w3t, m.w4t, m.master + L.b1, m.master + L.b2, m.master + L.b3, m.master + L.b4};
+    hipLaunchKernelGGL(k_mlp_infer, dim3(nblk(n, 128)), dim3(256), 0, s, X, W, Y, n);
+    return hipGetLastError();
+}
+
+// Forward through the GEMM path, keeping what the backward pass needs.  With T, also the loss
+// and the output-layer delta.
+hipError_t mlp_forward_train(Mlp& m, const uint16_t* X, const float* T, float* Y, uint32_t M, hipStream_t s) {
+    const MlpLayout L = mlp_layout();
+    const float* b = m.master;
+    hipError_t e;
+#define WC_TRY(x)                            \
+    do {                                     \
+        e = (x);                             \
+        if (e != hipSuccess) return e;       \
+    } while (0)
+    // layer 1..3
+    WC_TRY(gemm(X, kMlpInPad, 1, m.w1, kMlpHidden, 1, m.z[0], kMlpHidden, M, kMlpHidden, kMlpInPad, false, s));
+    hipLaunchKernelGGL(k_bias_silu, dim3(nblk((long)M * kMlpHidden, 256)), dim3(256), 0, s, m.z[0], b + L.b1, m.h[0],
+                       (int)M, kMlpHidden, kMlpHidden);
+    WC_TRY(hipGetLastError());
+    WC_TRY(gemm(m.h[0], kMlpHidden, 1, m.w2, kMlpHidden, 1, m.z[1], kMlpHidden, M, kMlpHidden, kMlpHidden, false, s));
+    hipLaunchKernelGGL(k_bias_silu, dim3(nblk((long)M * kMlpHidden, 256)), dim3(256), 0, s, m.z[1], b + L.b2, m.h[1],
+                       (int)M, kMlpHidden, kMlpHidden);
+    WC_TRY(hipGetLastError());
+    WC_TRY(gemm(m.h[1], kMlpHidden, 1, m.w3, kMlpHidden, 1, m.z[2], kMlpHidden, M, kMlpHidden, kMlpHidden, false, s));
+    hipLaunchKernelGGL(k_bias_silu, dim3(nblk((long)M * kMlpHidden, 256)), dim3(256), 0, s, m.z[2], b + L.b3, m.h[2],
+                       (int)M, kMlpHidden, kMlpHidden);
+    WC_TRY(hipGetLastError());
+    WC_TRY(gemm(m.h[2], kMlpHidden, 1, m.w4, kMlpOutPad, 1, m.yacc, kMlpOutPad, M, kMlpOutPad, kMlpHidden, false, s));
+    if (T) WC_TRY(hipMemsetAsync(m.loss, 0, sizeof(float), s));
+    const float scale = 2.0f / (3.0f * (float)M);
+    hipLaunchKernelGGL(k_output, dim3(nblk(M, 256)), dim3(256), 0, s, m.yacc, b + L.b4, T, Y, m.d4, scale,
+                       T ? m.loss : nullptr, (int)M);
+    WC_TRY(hipGetLastError());
+    return hipSuccess;
+}
+
+// Gradients of the mean squared error into m.grad (after mlp_forward_train with targets).
+hipError_t mlp_backward(Mlp& m, const uint16_t* X, uint32_t M, hipStream_t s) {
+    const MlpLayout L = mlp_layout();
+    float* g = m.grad;
+    hipError_t e;
+    WC_TRY(hipMemsetAsync(g, 0, L.total * sizeof(float), s));
+    // layer 4: dW4 = H3^T D4, db4 = colsum D4
+    WC_TRY(gemm(m.h[2], 1, kMlpHidden, m.d4, kMlpOutPad, 1, g + L.w4, kMlpOutPad, kMlpHidden, kMlpOutPad, M, true, s));
+    hipLaunchKernelGGL(k_colsum, dim3(kMlpOutPad), dim3(256), 0, s, m.d4, g + L.b4, (int)M, kMlpOut, kMlpOutPad);
+    WC_TRY(hipGetLastError());
+    // G3 = D4 W4^T, D3 = G3 * silu'(Z3)
+    WC_TRY(gemm(m.d4, kMlpOutPad, 1, m.w4, 1, kMlpOutPad, m.gbuf, kMlpHidden, M, kMlpHidden, kMlpOutPad, false, s));
+    hipLaunchKernelGGL(k_silu_backward, dim3(nblk((long)M * kMlpHidden, 256)), dim3(256), 0, s, m.gbuf, m.z[2], m.d,
+                       (long)M * kMlpHidden);
+    WC_TRY(hipGetLastError());
+    const size_t wofs[3] = {L.w1, L.w2, L.w3}, bofs[3] = {L.b1, L.b2, L.b3};
+    const uint16_t* wrm[3] = {m.w1, m.w2, m.w3};
+    for (int l = 2; l >= 0; --l) {
+        // dW_l = H_{l-1}^T D_l (H_{-1} = X), db_l = colsum D_l
+        const uint16_t* Hin = l == 0 ? X : m.h[l - 1];
+        const int kin = l == 0 ? kMlpInPad : kMlpHidden;
+        WC_TRY(gemm(Hin, 1, kin, m.d, kMlpHidden, 1, g + wofs[l], kMlpHidden, kin, kMlpHidden, M, true, s));
+        hipLaunchKernelGGL(k_colsum, dim3(kMlpHidden), dim3(256), 0, s, m.d, g + bofs[l], (int)M, kMlpHidden,
+                           kMlpHidden);
+        WC_TRY(hipGetLastError());
+        if (l == 0) break;
+        // G_{l-1} = D_l W_l^T, D_{l-1} = G * silu'(Z_{l-1})
+        WC_TRY(gemm(m.d, kMlpHidden, 1, wrm[l], 1, kMlpHidden, m.gbuf, kMlpHidden, M, kMlpHidden, kMlpHidden, false,
+                    s));
+        hipLaunchKernelGGL(k_silu_backward, dim3(nblk((long)M * kMlpHidden, 256)), dim3(256), 0, s, m.gbuf,
+                           m.z[l - 1], m.d, (long)M * kMlpHidden);
+        WC_TRY(hipGetLastError());
+    }
+    return hipSuccess;
+}
+
+hipError_t mlp_adam(Mlp& m, float lr, hipStream_t s) {
+    const MlpLayout L = mlp_layout();
+    m.adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - powf(b1, (float)m.adam_t), c2 = 1.0f - powf(b2, (float)m.adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(nblk((long)L.total, 256)), dim3(256), 0, s, m.master, m.grad, m.adam_m, m.adam_v,
+                       (int)L.total, lr, b1, b2, eps, c1, c2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return mlp_refresh_bf16(m, s);
+#undef WC_TRY
+}
+
+}  // namespace wc

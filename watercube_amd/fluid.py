@@ -132,6 +132,20 @@ class Fluid:
         t = L.wc_tuning(density_variant, update_variant)
         L.check(L.lib().wc_set_tuning(self.ctx, C.byref(t)), "wc_set_tuning")
 
+    def track_ids(self, enable: bool = True) -> None:
+        L.check(L.lib().wc_track_ids(self.ctx, 1 if enable else 0), "wc_track_ids")
+
+    def ids(self, which: int = 1) -> np.ndarray:
+        n = self.numParticles()
+        out = np.empty(n, np.uint32)
+        L.check(L.lib().wc_get_ids(self.ctx, which, L.u32ptr(out), n), "wc_get_ids")
+        return out
+
+    def max_density(self) -> float:
+        v = C.c_float()
+        L.check(L.lib().wc_max_density(self.ctx, C.byref(v)), "wc_max_density")
+        return float(v.value)
+
     def close(self) -> None:
         if self.ctx is not None:
             L.lib().wc_destroy(self.ctx)
