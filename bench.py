@@ -44,17 +44,29 @@ def algorithmic_bytes(n: int, ncells: int) -> dict:
     }
 
 
-def read_traffic(kernel: str) -> float | None:
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9  # one wave64 VALU instruction per CU per cycle (no packed math)
+
+
+def _profile_entry(kernel: str) -> dict:
     f = ROOT / "profiles" / f"traffic_{WORKLOAD}.json"
     if not f.exists():
-        return None
+        return {}
     try:
-        d = json.loads(f.read_text())
-        v = d["kernels"][kernel]["hbm_bytes_per_launch"]
-        return float(v)
-    except Exception:
-        return None
+        return dict(json.loads(f.read_text())["kernels"].get(kernel, {}))
+    except (ValueError, KeyError):
+        return {}
+
+
+def read_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
+    v = _profile_entry(kernel).get("hbm_bytes_per_launch")
+    return None if v is None else float(v)
+
+
+def read_valu(kernel: str) -> float | None:
+    """VALU wave instructions per launch of `kernel` (SQ_INSTS_VALU, committed summary)."""
+    v = _profile_entry(kernel).get("valu_wave_instr")
+    return None if v is None else float(v)
 
 
 def cpu_baseline(p, parts, dt, budget_s: float = 12.0, max_steps: int = 60) -> dict:
@@ -120,6 +132,58 @@ def batched_section(steps: int, warmup: int = 10, latency_samples: int = 20) -> 
     }
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int = 20) -> dict:
+    """The learned closure on the 1,000,000-particle block: simulated time per wall second of
+    plain steps at dt, plain steps at K*dt and judged learned steps at K*dt (K = 2, 4) over
+    `horizon` small steps from the same state, each run's per-particle density error against
+    the plain small-dt run (particles matched by identity), and the inference kernel's matrix
+    core utilisation (116,224 FLOP per particle over its measured duration, against the dense
+    bf16 peak)."""
+    import ctypes as C
+
+    from watercube_amd import Fluid
+    from watercube_amd import _lib as L
+    from watercube_amd import closure
+
+    net = closure.ClosureNet(max_batch=1024).load(weights)
+    res = closure.evaluate(net, WORKLOAD, horizon=horizon, ks=(2, 4), judge_tol=0.01,
+                           log=lambda m: print(f"# learned: {m}", file=sys.stderr, flush=True))
+    n = int(p.num_particles)
+    x = closure.DeviceArray((n, closure.NFEAT), np.uint16)
+    y = closure.DeviceArray((n, 4), np.float32)
+    with Fluid(p, parts) as f:
+        f.sort_.run()
+        f.runDensityProg()
+        f.synchronize()
+        feat_s = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            L.check(L.lib().wc_closure_features(f.ctx, C.c_float(4.0), None, n, x.ptr), "wc_closure_features")
+            feat_s.append(time.perf_counter() - t0)
+    ms = net.profile_predict(x, y, n, reps=reps)
+    x.free()
+    y.free()
+    flops = closure.mlp_flops_per_particle() * n
+    tflops = flops / (ms * 1e-3) / 1e12
+    res["inference"] = {"particles": n, "ms": ms, "tflops": tflops, "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                        "mfma_utilisation": tflops / MFMA_BF16_PEAK_TFLOPS,
+                        "flop_per_particle": closure.mlp_flops_per_particle()}
+    res["features_ms"] = 1e3 * float(np.median(feat_s))
+    res["weights"] = str(weights.relative_to(ROOT)) if weights.is_relative_to(ROOT) else str(weights)
+    best = None
+    for K in (2, 4):
+        e = res.get(f"K{K}", {}).get("learned")
+        if e and e["max_density_err"] <= 0.01 and (best is None or e["sim_s_per_s"] > best[1]["sim_s_per_s"]):
+            best = (K, e)
+    res["best_within_1pct"] = None if best is None else {
+        "K": best[0], "sim_s_per_s": best[1]["sim_s_per_s"], "speedup_vs_plain": best[1]["speedup_vs_plain"],
+        "particle_steps_per_s_equiv": best[1]["sim_s_per_s"] / dt * n}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +192,9 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch-steps", type=int, default=500, help="steps of the 256-scene batch (0 skips it)")
+    ap.add_argument("--weights", default=str(ROOT / "watercube_amd" / "weights" / "closure.npy"),
+                    help="closure network weights for the learned section (skipped if absent)")
+    ap.add_argument("--learned-horizon", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step runs on a single MI355X)",
@@ -172,6 +239,11 @@ def main() -> int:
     print("# per-launch ms: " + ", ".join(f"{k}={v:.4f}" for k, v in per_launch_ms.items()), file=sys.stderr,
           flush=True)
 
+    valu = read_valu(dom)
+    valu_section = None if valu is None else {
+        "wave_instr_per_launch": valu, "achieved": valu / (per_launch_ms[dom] * 1e-3),
+        "peak": VALU_PEAK_WAVE_INSTR_PER_S, "unit": "wave-instr/s",
+        "frac": valu / (per_launch_ms[dom] * 1e-3) / VALU_PEAK_WAVE_INSTR_PER_S}
     out = {
         "metric": "particle-steps/s at 1M particles: plain vs learned large-dt, ≤1% density err",
         "value": n * args.steps / elapsed,
@@ -200,11 +272,19 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "algorithmic_bytes": algo[dom],
+            "binding": "valu",
+            "valu": valu_section,
         },
     }
     if args.batch_steps > 0:
         out["batched"] = batched_section(args.batch_steps)
         print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
+    weights = Path(args.weights)
+    if weights.exists():
+        out["learned"] = learned_section(p, parts, dt, weights, horizon=args.learned_horizon)
+        print(f"# learned: {json.dumps(out['learned'])}", file=sys.stderr, flush=True)
+    else:
+        print(f"# learned section skipped: no weights at {weights}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, parts, dt)
     print(json.dumps(out), flush=True)

@@ -397,3 +397,21 @@ def test_planted_mistake_is_caught(oracle):
                 rho = rho + m * (((a * a) * a) * c6)
         differs += ps["density"][i].tobytes() != f(rho).tobytes()
     assert differs > 0
+
+
+def test_cutoff_by_squared_distance_is_exact():
+    """The kernels accept a neighbour when d2 < cut2, with cut2 the smallest float whose
+    correctly rounded sqrt is >= h; the reference tests length(r) >= h (density.comp:117,
+    update.comp:169).  sqrtf is monotone, so the two agree for every float d2: checked on
+    every float within 4096 ulp of cut2 for each preset's h."""
+    f32 = np.float32
+    for h in (0.03, 0.016, 0.004, 0.04, 0.0123):
+        h = f32(h)
+        cut2 = f32(h * h)
+        while np.sqrt(np.nextafter(cut2, f32(0))) >= h:
+            cut2 = np.nextafter(cut2, f32(0))
+        while np.sqrt(cut2) < h:
+            cut2 = np.nextafter(cut2, f32(np.inf))
+        bits = cut2.view(np.uint32).astype(np.int64) + np.arange(-4096, 4097)
+        d2 = bits.astype(np.uint32).view(np.float32)
+        assert np.array_equal(np.sqrt(d2) >= h, d2 >= cut2)
