@@ -94,9 +94,18 @@ def _enumeration(oracle, p, pos, counts, offsets, t):
     return out
 
 
-def test_features_match_numpy_reference(oracle):
-    p, parts, dt, _ = scenes.workload("dam_break_16k")
+@pytest.mark.parametrize("scene", ["dam_break_16k", "dense_h3"])
+def test_features_match_numpy_reference(scene, oracle):
+    """dam_break_16k has about 33 neighbours inside h per particle (the 32-of-n selection at the
+    end of the walk); dense_h3 (h = 3 spacings) about 113, so the kernel's list also overflows
+    and is cut back to the 32 nearest during the walk."""
+    if scene == "dam_break_16k":
+        p, parts, dt, _ = scenes.workload("dam_break_16k")
+    else:
+        p, parts = scenes.scaled_block((24, 24, 24), 0.01, seed=5, h_over_s=3.0)
+        dt = 2e-4
     K = 4.0
+    print(f"features {scene}: {p.num_particles} particles", flush=True)
     with Fluid(p, parts) as f:
         f.step(dt, 30)
         f.sort_.run()
@@ -125,8 +134,9 @@ def test_features_match_numpy_reference(oracle):
             d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
             if not (d2 >= cut2):
                 cand.append((d2, e, u))
-        cand.sort(key=lambda c: (c[0], c[1]))
-        cand = cand[:32]
+        # the 32 nearest (ties: earlier in enumeration order), listed in enumeration order
+        cand.sort(key=lambda c: (c[0].view(np.uint32), c[1]))
+        cand = sorted(cand[:32], key=lambda c: c[1])
         want = np.zeros(208, np.float32)
         for k, (_, _, u) in enumerate(cand):
             want[6 * k:6 * k + 3] = (pos[u] - pos[t]) * inv_h

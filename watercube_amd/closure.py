@@ -161,6 +161,13 @@ def training_scene(n_target: int, kind: str, seed: int):
         parts = S.block(2 * nz, 2 * nz, nz, s, 0.1 * s, seed)
     elif kind == "ball":
         parts = S.sphere(n_target, (0.5, 0.6, 0.5), s, 0.1 * s, seed)
+    elif kind == "tall_column":
+        # full height of the 1,000,000-particle block (100 layers), narrower
+        nx = max(2, round(math.sqrt(n_target / 100)))
+        parts = S.block(nx, 100, nx, s, 0.1 * s, seed)
+    elif kind == "corner_cube":
+        nx = max(2, round(n_target ** (1 / 3)))
+        parts = S.block(nx, nx, nx, s, 0.1 * s, seed)
     else:
         nx = max(2, round((n_target / 4) ** (1 / 3)))
         a = S.block(nx, 2 * nx, nx, s, 0.1 * s, seed)
@@ -205,11 +212,11 @@ def make_training_set(num_traj: int = 200, steps: int = 400, every: int = 20, pe
                       seed: int = 0, ks=(2, 4), log=print):
     rng = np.random.default_rng(seed)
     xs, ts = [], []
-    kinds = ("block", "ball", "two_columns")
+    kinds = ("block", "ball", "two_columns", "tall_column", "corner_cube")
     t0 = time.perf_counter()
     for j in range(num_traj):
         n_target = int(round(math.exp(rng.uniform(math.log(64 * 1024), math.log(262144)))))
-        kind = kinds[j % 3]
+        kind = kinds[j % len(kinds)]
         p, parts = training_scene(n_target, kind, seed * 100003 + j)
         n = p.num_particles
         dt = 1e-4
@@ -293,6 +300,65 @@ def _densities_by_id(f: Fluid) -> np.ndarray:
     return out
 
 
+def _state_by_id(f: Fluid) -> np.ndarray:
+    ids = f.ids(1)
+    rec = f.particles(1)
+    out = np.empty_like(rec)
+    out[ids] = rec
+    return out
+
+
+def one_step_errors(net: ClosureNet, p, parts: np.ndarray, dt: float, K: int, starts=(0, 30, 60, 90),
+                    variants=None) -> dict:
+    """From states along a plain small-dt run: one plain step of K*dt and one learned step of K*dt
+    (judge disabled) against K plain steps of dt.  RMS velocity and position errors over the
+    particles at least 2h from every wall, per start and averaged."""
+    h = float(p.kernel_radius)
+    size = float(p.size)
+
+    def fluid(state):
+        f = Fluid(p, state).setup()
+        f.track_ids(True)
+        if variants:
+            f.set_tuning(*variants)
+        return f
+
+    rows = []
+    with fluid(parts) as run:
+        done = 0
+        for s0 in starts:
+            if s0 > done:
+                run.step(dt, s0 - done)
+                done = s0
+            state = _state_by_id(run)
+            pos0 = state["position"]
+            interior = np.all((pos0 > 2 * h) & (pos0 < size - 2 * h), axis=1)
+            with fluid(state) as a:
+                a.step(dt, K)
+                ref = _state_by_id(a)
+            with fluid(state) as b:
+                b.step(dt * K, 1)
+                big = _state_by_id(b)
+            with fluid(state) as c:
+                st = L.wc_closure_stats()
+                L.check(L.lib().wc_closure_run(c.ctx, net.h, C.c_float(dt), K, 1, C.c_float(1e30), C.byref(st)),
+                        "wc_closure_run")
+                lrn = _state_by_id(c)
+
+            def rms(x, field):
+                d = (x[field] - ref[field])[interior]
+                return float(np.sqrt(np.mean(np.sum(d.astype(np.float64) ** 2, axis=1))))
+
+            rows.append({"start": s0, "interior": int(interior.sum()),
+                         "plain_large": {"vel_rms": rms(big, "velocity"), "pos_rms": rms(big, "position")},
+                         "learned": {"vel_rms": rms(lrn, "velocity"), "pos_rms": rms(lrn, "position")}})
+    mean = {k: {m: float(np.mean([r[k][m] for r in rows])) for m in ("vel_rms", "pos_rms")}
+            for k in ("plain_large", "learned")}
+    return {"K": K, "per_start": rows, "mean": mean,
+            "vel_err_ratio": mean["learned"]["vel_rms"] / max(mean["plain_large"]["vel_rms"], 1e-30),
+            "pos_err_ratio": mean["learned"]["pos_rms"] / max(mean["plain_large"]["pos_rms"], 1e-30)}
+
+
 def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 100, ks=(2, 4),
              judge_tol: float = 0.01, log=print) -> dict:
     """Plain small-dt run vs learned large steps vs plain large steps over `horizon` small steps
@@ -345,6 +411,8 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
                                 "fallbacks": int(st.fallbacks), "wall_s": wall, "sim_s_per_s": macro * K * dt / wall,
                                 "speedup_vs_plain": (macro * K * dt / wall) / res["plain"]["sim_s_per_s"],
                                 "max_density_jump": float(st.max_density_jump)}
+        if net is not None:
+            entry["one_step"] = one_step_errors(net, p, parts, dt, K, variants=variants)
         res[f"K{K}"] = entry
         log(f"K={K}: {json.dumps(entry)}")
     return res
@@ -359,7 +427,7 @@ def main(argv=None) -> int:
     a.add_argument("--steps", type=int, default=400)
     a.add_argument("--every", type=int, default=20)
     a.add_argument("--per-snapshot", type=int, default=1024)
-    a.add_argument("--epochs", type=int, default=4)
+    a.add_argument("--epochs", type=int, default=8)
     a.add_argument("--batch", type=int, default=8192)
     a.add_argument("--lr", type=float, default=2e-3)
     a.add_argument("--log", default=None)

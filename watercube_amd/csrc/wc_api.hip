@@ -301,10 +301,10 @@ int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
         if (!ok) return;
         if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) ok = false;
     };
-    alloc((void**)&b.posrho[0], n * sizeof(float4));
-    alloc((void**)&b.posrho[1], n * sizeof(float4));
-    alloc((void**)&b.velp[0], n * sizeof(float4));
-    alloc((void**)&b.velp[1], n * sizeof(float4));
+    alloc((void**)&b.posrho[0], (n + wc::kRecordPad) * sizeof(float4));
+    alloc((void**)&b.posrho[1], (n + wc::kRecordPad) * sizeof(float4));
+    alloc((void**)&b.velp[0], (n + wc::kRecordPad) * sizeof(float4));
+    alloc((void**)&b.velp[1], (n + wc::kRecordPad) * sizeof(float4));
     alloc((void**)&b.key, n * 4);
     alloc((void**)&b.rank, n * 4);
     alloc((void**)&b.slot_tmp, n * 4);
@@ -341,10 +341,10 @@ int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
     zero(b.skey, n * 4);
     zero(b.key, n * 4);
     zero(b.slot_tmp, n * 4);
-    zero(b.posrho[0], n * sizeof(float4));
-    zero(b.posrho[1], n * sizeof(float4));
-    zero(b.velp[0], n * sizeof(float4));
-    zero(b.velp[1], n * sizeof(float4));
+    zero(b.posrho[0], (n + wc::kRecordPad) * sizeof(float4));
+    zero(b.posrho[1], (n + wc::kRecordPad) * sizeof(float4));
+    zero(b.velp[0], (n + wc::kRecordPad) * sizeof(float4));
+    zero(b.velp[1], (n + wc::kRecordPad) * sizeof(float4));
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         destroy_ctx(c);

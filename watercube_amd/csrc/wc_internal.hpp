@@ -20,6 +20,9 @@ constexpr int kScanItems = 16;        // counts per scan thread (four 16-byte lo
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 constexpr uint32_t kLookbackSpinLimit = 1u << 24;  // bounded wait in the scan look-back
+// Records allocated (and zeroed) past the last particle of the particle buffers: the neighbour
+// passes load a cell's slots CHUNK at a time and may read up to CHUNK - 1 records beyond it.
+constexpr size_t kRecordPad = 8;
 
 // Per-scene constants, precomputed on the host in float so the device sees exactly the bits the
 // CPU transcription uses.

@@ -69,9 +69,10 @@ __device__ __forceinline__ uint32_t total_blocks(const Scenes& S) {
 }
 
 // Walks the in-grid neighbour cells of scene-local cell `key` in NEIGHBORHOOD order
-// (density.comp:40-50: dx outermost, dz innermost); visit(start, end) receives each cell's slot
-// range (clipped to `limit`) and end_plane() runs after every dx plane.  The cell loop is
-// wave-uniform.
+// (density.comp:40-50: dx outermost, dz innermost); visit(start, end, own) receives each cell's
+// slot range (clipped to `limit`) and whether it is the particle's own cell (the 14th of the 27,
+// the same iteration in every lane: std::true_type there, std::false_type elsewhere), and
+// end_plane() runs after every dx plane.  The trip counts of both cell loops are wave-uniform.
 template <typename Visit, typename EndPlane>
 __device__ __forceinline__ void for_each_neighbour_cell(uint32_t key, const uint2* __restrict__ range,
                                                         const StepConsts& c, uint32_t limit, Visit& visit,
@@ -89,7 +90,10 @@ __device__ __forceinline__ void for_each_neighbour_cell(uint32_t key, const uint
             const int nz = cz + (dyz % 3 - 1);
             if (nx < 0 || ny < 0 || nz < 0 || nx >= R || ny >= R || nz >= R) continue;
             const uint2 rg = range[(nz * R + ny) * R + nx];
-            visit(rg.x, min(rg.x + rg.y, limit));
+            if (dx == 0 && dyz == 4)
+                visit(rg.x, min(rg.x + rg.y, limit), std::true_type{});
+            else
+                visit(rg.x, min(rg.x + rg.y, limit), std::false_type{});
         }
         end_plane();
     }
@@ -354,36 +358,47 @@ __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t*
 // in enumeration order, so every sum is formed exactly as the reference forms it, and the
 // result does not depend on BLOCK, CAP, CHUNK or PLANE_DRAIN (the tuning knobs of the passes).
 // ------------------------------------------------------------------------------------------
+// Pointers into LDS (32-bit addresses, so list arithmetic stays 32-bit).
+template <typename T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+
 template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, typename Entry, typename Drain>
 __device__ __forceinline__ void search_neighbours(uint32_t key, uint32_t t, uint32_t limit, float px, float py,
                                                   float pz, const float4* __restrict__ pos,
                                                   const uint2* __restrict__ range, const StepConsts& c,
-                                                  Entry* __restrict__ list, Drain& drain_fn) {
+                                                  lds_ptr<Entry> list, Drain& drain_fn) {
     static_assert(CAP >= CHUNK, "list must hold a chunk");
-    int cnt = 0;
+    static_assert(CHUNK <= (int)kRecordPad + 1, "chunked loads may run CHUNK - 1 records past a cell");
+    // `next` is the lane's next free list entry; the list holds (next - list) / BLOCK entries
+    lds_ptr<Entry> next = list;
+    const lds_ptr<Entry> full = list + (CAP - CHUNK) * BLOCK;  // past this a chunk might not fit
     auto drain = [&]() {
-        drain_fn(cnt);
-        cnt = 0;
+        drain_fn((int)((next - list) / BLOCK));
+        next = list;
     };
-    auto visit = [&](uint32_t start, uint32_t end) {
+    auto visit = [&](uint32_t start, uint32_t end, auto own) {
         for (uint32_t u = start; u < end; u += CHUNK) {
-            if (cnt > CAP - CHUNK) drain();
-            const uint32_t last = end - 1;
-            float4 q[CHUNK];
+            if (next > full) drain();
+            const uint32_t left = end - u;  // slots of the cell from u on (>= 1 here): u + j is one iff j < left
+            // slots past the cell are loaded (the buffers are padded) and rejected below
+            const float4* q = pos + u;
+            float4 r[CHUNK];
 #pragma unroll
-            for (int j = 0; j < CHUNK; ++j) q[j] = pos[min(u + (uint32_t)j, last)];
+            for (int j = 0; j < CHUNK; ++j) r[j] = q[j];
 #pragma unroll
             for (int j = 0; j < CHUNK; ++j) {
-                const uint32_t v = u + (uint32_t)j;
-                const float rx = px - q[j].x, ry = py - q[j].y, rz = pz - q[j].z;
+                const float rx = px - r[j].x, ry = py - r[j].y, rz = pz - r[j].z;
                 const float d2 = (rx * rx + ry * ry) + rz * rz;
                 // dist < h  <=>  !(d2 >= cut2), NaN included (sqrtf(NaN) >= h is false)
-                const bool ok = (v < end) & (v != t) & !(d2 >= c.cut2);
+                bool ok = ((uint32_t)j < left) & !(d2 >= c.cut2);
+                if constexpr (decltype(own)::value) ok &= (u + (uint32_t)j != t);
                 if constexpr (std::is_same<Entry, float>::value)
-                    list[cnt * BLOCK] = d2;  // density: squared distance
+                    *next = d2;  // density: squared distance
                 else
-                    list[cnt * BLOCK] = v;   // force pass: the neighbour's slot
-                cnt += ok ? 1 : 0;
+                    *next = u + (uint32_t)j;  // force pass: the neighbour's slot
+                next += ok ? BLOCK : 0;
+                // keep `next` an address in one register (not base + offset: one add fewer per candidate)
+                asm("" : "+v"(next));
             }
         }
     };
@@ -429,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(Scenes S, float4* __restrict_
     if (lt >= c.n) return;
     const uint32_t t = d.p_off + lt;
     const float4 p = posrho[t];
-    float* list = s_list + threadIdx.x;
+    const lds_ptr<float> list = (lds_ptr<float>)s_list + threadIdx.x;
     float rho = c.m * poly6(c, 0.0f);
     auto drain = [&](int cnt) {
         for (int k = 0; k < cnt; ++k) rho += c.m * poly6(c, sqrtf(list[k * BLOCK]));
@@ -614,12 +629,19 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
     const uint32_t t = d.p_off + lt;
     const float4 p = posrho_in[t];
     const float4 pv = velp_in[t];
-    uint32_t* list = s_list + threadIdx.x;
+    const lds_ptr<uint32_t> list = (lds_ptr<uint32_t>)s_list + threadIdx.x;
     ForceSums f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // neighbour k + 1's records are loaded while neighbour k's terms are formed
     auto drain = [&](int cnt) {
+        if (cnt <= 0) return;
+        uint32_t u = list[0];
+        float4 q = posrho_in[u], qv = velp_in[u];
         for (int k = 0; k < cnt; ++k) {
-            const uint32_t u = list[k * BLOCK];
-            f.add(c, p, pv, posrho_in[u], velp_in[u]);
+            const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
+            const float4 qn = posrho_in[un], qvn = velp_in[un];
+            f.add(c, p, pv, q, qv);
+            q = qn;
+            qv = qvn;
         }
     };
     search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho_in,
@@ -655,12 +677,39 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
 
 // ------------------------------------------------------------------------------------------
 // Learned closure features (single-scene contexts).  For every requested slot of the sorted
-// buffer: the 32 nearest neighbours inside the kernel radius (the density pass's acceptance
-// test; ascending squared distance, ties in NEIGHBORHOOD enumeration order), each as
-// ((x_j - x_i) / h, (v_j - v_i) / v_ref), then rho_i / rho0 - 1 scaled by 10, the step
-// multiplier / 4 and the fraction of the 32 places filled; zero padded to 208 bf16 values.
+// buffer: its 32 nearest neighbours inside the kernel radius (the density pass's acceptance
+// test; nearest by squared distance, among equal distances the earlier in NEIGHBORHOOD
+// enumeration order), listed in enumeration order, each as ((x_j - x_i) / h, (v_j - v_i) / v_ref);
+// then rho_i / rho0 - 1 scaled by 10, the step multiplier / 4 and the fraction of the 32 places
+// filled; zero padded to 208 bf16 values.
 // ------------------------------------------------------------------------------------------
 constexpr int kFeatBlock = 64;
+constexpr int kFeatCap = 40;    // accepted neighbours held per particle between selections
+constexpr int kFeatChunk = 4;   // candidate slots loaded per step of a lane's walk
+
+// Removes the `drop` farthest of a lane's `cnt` listed neighbours (largest squared distance;
+// of equal ones the later), keeping the rest in order.  Squared distances compare as their bit
+// patterns: non-negative floats order like the integers, and NaN, which the acceptance test
+// lets through, ranks above every number.
+__device__ __forceinline__ int drop_farthest(lds_ptr<uint32_t> key, lds_ptr<uint32_t> slot, int cnt, int drop) {
+    for (int m = 0; m < drop; ++m) {
+        uint32_t best = 0;
+        int at = 0;
+        for (int k = 0; k < cnt; ++k) {
+            const uint32_t v = key[k * kFeatBlock];
+            if (v >= best) {
+                best = v;
+                at = k;
+            }
+        }
+        for (int k = at; k + 1 < cnt; ++k) {
+            key[k * kFeatBlock] = key[(k + 1) * kFeatBlock];
+            slot[k * kFeatBlock] = slot[(k + 1) * kFeatBlock];
+        }
+        --cnt;
+    }
+    return cnt;
+}
 
 __device__ __forceinline__ uint32_t bf16_bits(float f) {
     uint32_t u = __float_as_uint(f);
@@ -676,8 +725,10 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
                                                                  const uint2* __restrict__ range,
                                                                  const uint32_t* __restrict__ rows, uint32_t nrows,
                                                                  float dt_mult, float inv_v, uint16_t* __restrict__ X) {
-    __shared__ float s_d2[kClosureNeighbours * kFeatBlock];
-    __shared__ uint32_t s_slot[kClosureNeighbours * kFeatBlock];
+    static_assert(kFeatChunk <= (int)kRecordPad + 1, "chunked loads may run kFeatChunk - 1 records past a cell");
+    static_assert(kFeatCap >= kClosureNeighbours + kFeatChunk, "a chunk must fit after a selection");
+    __shared__ uint32_t s_key[kFeatCap * kFeatBlock];
+    __shared__ uint32_t s_slot[kFeatCap * kFeatBlock];
     const uint32_t r = blockIdx.x * kFeatBlock + threadIdx.x;
     if (r >= nrows) return;
     const SceneDesc& d = S.desc[0];
@@ -690,30 +741,32 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
     }
     const float4 p = posrho[t];
     const float4 pv = velp[t];
-    float* l_d2 = s_d2 + threadIdx.x;
-    uint32_t* l_slot = s_slot + threadIdx.x;
+    const lds_ptr<uint32_t> key = (lds_ptr<uint32_t>)s_key + threadIdx.x;
+    const lds_ptr<uint32_t> slot = (lds_ptr<uint32_t>)s_slot + threadIdx.x;
     int cnt = 0;
-    auto visit = [&](uint32_t start, uint32_t end) {
-        for (uint32_t u = start; u < end; ++u) {
-            if (u == t) continue;
-            const float4 q = posrho[u];
-            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-            const float d2 = (rx * rx + ry * ry) + rz * rz;
-            if (d2 >= c.cut2) continue;
-            if (cnt == kClosureNeighbours && !(d2 < l_d2[(kClosureNeighbours - 1) * kFeatBlock])) continue;
-            int pos = cnt < kClosureNeighbours ? cnt : kClosureNeighbours - 1;
-            while (pos > 0 && l_d2[(pos - 1) * kFeatBlock] > d2) {
-                l_d2[pos * kFeatBlock] = l_d2[(pos - 1) * kFeatBlock];
-                l_slot[pos * kFeatBlock] = l_slot[(pos - 1) * kFeatBlock];
-                --pos;
+    auto visit = [&](uint32_t start, uint32_t end, auto own) {
+        for (uint32_t u = start; u < end; u += kFeatChunk) {
+            if (cnt > kFeatCap - kFeatChunk) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
+            const uint32_t left = end - u;
+            const float4* q = posrho + u;  // the buffers are padded past the last particle
+            float4 v[kFeatChunk];
+#pragma unroll
+            for (int j = 0; j < kFeatChunk; ++j) v[j] = q[j];
+#pragma unroll
+            for (int j = 0; j < kFeatChunk; ++j) {
+                const float rx = p.x - v[j].x, ry = p.y - v[j].y, rz = p.z - v[j].z;
+                const float d2 = (rx * rx + ry * ry) + rz * rz;
+                bool ok = ((uint32_t)j < left) & !(d2 >= c.cut2);
+                if constexpr (decltype(own)::value) ok &= (u + (uint32_t)j != t);
+                key[cnt * kFeatBlock] = __float_as_uint(d2);
+                slot[cnt * kFeatBlock] = u + (uint32_t)j;
+                cnt += ok ? 1 : 0;
             }
-            l_d2[pos * kFeatBlock] = d2;
-            l_slot[pos * kFeatBlock] = u;
-            if (cnt < kClosureNeighbours) ++cnt;
         }
     };
     auto nothing = []() {};
     for_each_neighbour_cell(skey[t], range + d.c_off, c, d.p_off + c.n, visit, nothing);
+    if (cnt > kClosureNeighbours) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
 
     const float inv_h = 1.0f / c.h;
     // eight groups of four neighbours: 24 values = three 16-byte stores
@@ -723,7 +776,7 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
         for (int j = 0; j < 4; ++j) {
             const int k = g * 4 + j;
             if (k < cnt) {
-                const uint32_t u = l_slot[k * kFeatBlock];
+                const uint32_t u = slot[k * kFeatBlock];
                 const float4 q = posrho[u];
                 const float4 qv = velp[u];
                 v[6 * j + 0] = (q.x - p.x) * inv_h;

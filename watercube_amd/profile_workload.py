@@ -27,6 +27,9 @@ def main() -> int:
     ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
     ap.add_argument("--tuning", default="auto",
                     help="'auto' runs the autotuner first; 'D,U' fixes the density / force-pass variants")
+    ap.add_argument("--closure-k", type=int, default=0,
+                    help="K > 0: after the plain steps, run --steps judged learned steps of K*dt "
+                         "(wc_closure_run with the default weights)")
     args = ap.parse_args()
 
     from watercube_amd import Fluid, scenes
@@ -52,7 +55,22 @@ def main() -> int:
                 f.runUpdateProg(dt)
         f.synchronize()
         dt_wall = time.perf_counter() - t0
-    print(f"profile_workload: {args.steps} {args.mode} steps in {dt_wall * 1e3:.2f} ms", flush=True)
+        print(f"profile_workload: {args.steps} {args.mode} steps in {dt_wall * 1e3:.2f} ms", flush=True)
+        if args.closure_k > 0:
+            import ctypes as C
+
+            from watercube_amd import _lib as L
+            from watercube_amd.closure import DEFAULT_WEIGHTS, ClosureNet
+
+            net = ClosureNet(max_batch=1024).load(DEFAULT_WEIGHTS)
+            st = L.wc_closure_stats()
+            t0 = time.perf_counter()
+            L.check(L.lib().wc_closure_run(f.ctx, net.h, C.c_float(dt), args.closure_k, args.steps,
+                                           C.c_float(0.01), C.byref(st)), "wc_closure_run")
+            wall = time.perf_counter() - t0
+            print(f"profile_workload: {st.macro_steps} learned steps of {args.closure_k} dt in {wall * 1e3:.2f} ms "
+                  f"(device {st.wall_ms:.2f} ms, {st.fallbacks} fallbacks)", flush=True)
+            net.close()
     return 0
 
 
