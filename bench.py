@@ -136,21 +136,30 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int = 20) -> dict:
-    """The learned closure on the 1,000,000-particle block: simulated time per wall second of
-    plain steps at dt, plain steps at K*dt and judged learned steps at K*dt (K = 2, 4) over
-    `horizon` small steps from the same state, each run's per-particle density error against
-    the plain small-dt run (particles matched by identity), and the inference kernel's matrix
-    core utilisation (116,224 FLOP per particle over its measured duration, against the dense
-    bf16 peak)."""
+    """Large steps on the 1,000,000-particle block.  Over `horizon` small steps of dt from the same
+    state: plain steps of dt (the reference run), plain steps of K*dt, and judged learned steps
+    of K*dt (K = 2, 4; the network's correction averaged over neighbours, and also without that
+    averaging).  For each: simulated time per wall second and the per-particle density error
+    |rho - rho_ref| / rho0 against the reference run (particles matched by identity).  Then the
+    fastest method within 1% density error, by the mean and by the maximum over particles, in
+    particle-steps of dt per second; and the inference kernel's matrix-core utilisation (116,224
+    FLOP per particle over its measured duration, against the dense bf16 peak)."""
     import ctypes as C
 
     from watercube_amd import Fluid
     from watercube_amd import _lib as L
     from watercube_amd import closure
 
+    log = lambda m: print(f"# learned: {m}", file=sys.stderr, flush=True)  # noqa: E731
     net = closure.ClosureNet(max_batch=1024).load(weights)
-    res = closure.evaluate(net, WORKLOAD, horizon=horizon, ks=(2, 4), judge_tol=0.01,
-                           log=lambda m: print(f"# learned: {m}", file=sys.stderr, flush=True))
+    res = closure.evaluate(net, WORKLOAD, horizon=horizon, ks=(2, 4), judge_tol=0.01, smooth=True, log=log)
+    raw = closure.evaluate(net, WORKLOAD, horizon=horizon, ks=(2, 4), judge_tol=0.01, smooth=False, log=log)
+    for K in (2, 4):
+        e = raw[f"K{K}"]["learned"]
+        res[f"K{K}"]["learned_unsmoothed"] = {k: e[k] for k in ("mean_density_err", "p99_density_err",
+                                                                "max_density_err", "sim_s_per_s",
+                                                                "speedup_vs_plain", "fallbacks")}
+        res[f"K{K}"]["learned_unsmoothed"]["one_step_vel_err_ratio"] = raw[f"K{K}"]["one_step"]["vel_err_ratio"]
     n = int(p.num_particles)
     x = closure.DeviceArray((n, closure.NFEAT), np.uint16)
     y = closure.DeviceArray((n, 4), np.float32)
@@ -173,14 +182,23 @@ def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int =
                         "flop_per_particle": closure.mlp_flops_per_particle()}
     res["features_ms"] = 1e3 * float(np.median(feat_s))
     res["weights"] = str(weights.relative_to(ROOT)) if weights.is_relative_to(ROOT) else str(weights)
-    best = None
+
+    # candidates: (method, K, entry); the plain small-dt run has zero error by definition
+    cands = [("plain", 1, {"mean_density_err": 0.0, "max_density_err": 0.0, **res["plain"],
+                            "speedup_vs_plain": 1.0})]
     for K in (2, 4):
-        e = res.get(f"K{K}", {}).get("learned")
-        if e and e["max_density_err"] <= 0.01 and (best is None or e["sim_s_per_s"] > best[1]["sim_s_per_s"]):
-            best = (K, e)
-    res["best_within_1pct"] = None if best is None else {
-        "K": best[0], "sim_s_per_s": best[1]["sim_s_per_s"], "speedup_vs_plain": best[1]["speedup_vs_plain"],
-        "particle_steps_per_s_equiv": best[1]["sim_s_per_s"] / dt * n}
+        for m in ("plain_large", "learned", "learned_unsmoothed"):
+            if m in res[f"K{K}"]:
+                cands.append((m, K, res[f"K{K}"][m]))
+
+    def best(stat):
+        ok = [c for c in cands if c[2][stat] <= 0.01]
+        m, K, e = max(ok, key=lambda c: c[2]["sim_s_per_s"])
+        return {"method": m, "K": K, "particle_steps_per_s": e["sim_s_per_s"] / dt * n,
+                "speedup_vs_plain": e["speedup_vs_plain"], "mean_density_err": e["mean_density_err"],
+                "max_density_err": e["max_density_err"]}
+
+    res["best_within_1pct"] = {"by_mean_error": best("mean_density_err"), "by_max_error": best("max_density_err")}
     return res
 
 

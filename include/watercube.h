@@ -313,6 +313,12 @@ WC_API int wc_closure_features(wc_ctx* ctx, float dt_mult, const uint32_t* rows,
  * applies). */
 WC_API int wc_closure_update(wc_ctx* ctx, float dt, const void* corr_dev, float corr_scale, void* acc_dev);
 
+/* wc_closure_run applies, per particle, the average of the network corrections of the particle
+ * and its accepted neighbours with weights h - r (h for the particle itself; neighbours within
+ * 2h of a wall count with zero) (enable != 0, the default), or its own correction alone
+ * (enable = 0). */
+WC_API int wc_closure_set_smoothing(wc_ctx* ctx, int enable);
+
 typedef struct wc_closure_stats {
     uint32_t macro_steps;    /* learned steps of K * dt taken */
     uint32_t fallbacks;      /* of them replaced by K plain steps */

@@ -309,7 +309,7 @@ def _state_by_id(f: Fluid) -> np.ndarray:
 
 
 def one_step_errors(net: ClosureNet, p, parts: np.ndarray, dt: float, K: int, starts=(0, 30, 60, 90),
-                    variants=None) -> dict:
+                    variants=None, smooth: bool = True) -> dict:
     """From states along a plain small-dt run: one plain step of K*dt and one learned step of K*dt
     (judge disabled) against K plain steps of dt.  RMS velocity and position errors over the
     particles at least 2h from every wall, per start and averaged."""
@@ -340,6 +340,7 @@ def one_step_errors(net: ClosureNet, p, parts: np.ndarray, dt: float, K: int, st
                 b.step(dt * K, 1)
                 big = _state_by_id(b)
             with fluid(state) as c:
+                L.check(L.lib().wc_closure_set_smoothing(c.ctx, int(smooth)), "wc_closure_set_smoothing")
                 st = L.wc_closure_stats()
                 L.check(L.lib().wc_closure_run(c.ctx, net.h, C.c_float(dt), K, 1, C.c_float(1e30), C.byref(st)),
                         "wc_closure_run")
@@ -360,7 +361,7 @@ def one_step_errors(net: ClosureNet, p, parts: np.ndarray, dt: float, K: int, st
 
 
 def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 100, ks=(2, 4),
-             judge_tol: float = 0.01, log=print) -> dict:
+             judge_tol: float = 0.01, smooth: bool = True, log=print) -> dict:
     """Plain small-dt run vs learned large steps vs plain large steps over `horizon` small steps
     from the same state: per-particle density error against the small-dt run (particles matched
     by identity) and simulated time per wall-clock second.  Every run uses the same tuned
@@ -379,7 +380,7 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
         return f
 
     res = {"workload": workload, "particles": n, "dt": dt, "horizon_steps": horizon,
-           "judge_tol": judge_tol, "tuning": [int(v) for v in variants]}
+           "judge_tol": judge_tol, "smooth": bool(smooth), "tuning": [int(v) for v in variants]}
     with fluid() as f:
         f.step(dt, 1)  # captures the step graph
         f.synchronize()
@@ -393,13 +394,21 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
     for K in ks:
         macro = horizon // K
         with fluid() as f:
-            f.step(dt * K, macro)
+            f.step(dt * K, 1)  # captures the step graph
+            f.synchronize()
+            t0 = time.perf_counter()
+            f.step(dt * K, macro - 1)
+            f.synchronize()
+            t_large = (time.perf_counter() - t0) * macro / max(1, macro - 1)
             rho = _densities_by_id(f)
         err = np.abs(rho - ref) / rho0
         entry = {"plain_large": {"mean_density_err": float(err.mean()), "p99_density_err": float(np.percentile(err, 99)),
-                                 "max_density_err": float(err.max())}}
+                                 "max_density_err": float(err.max()), "wall_s": t_large,
+                                 "sim_s_per_s": macro * K * dt / t_large,
+                                 "speedup_vs_plain": (macro * K * dt / t_large) / res["plain"]["sim_s_per_s"]}}
         if net is not None:
             with fluid() as f:
+                L.check(L.lib().wc_closure_set_smoothing(f.ctx, int(smooth)), "wc_closure_set_smoothing")
                 st = L.wc_closure_stats()
                 L.check(L.lib().wc_closure_run(f.ctx, net.h, C.c_float(dt), K, macro, C.c_float(judge_tol),
                                                C.byref(st)), "wc_closure_run")
@@ -412,7 +421,7 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
                                 "speedup_vs_plain": (macro * K * dt / wall) / res["plain"]["sim_s_per_s"],
                                 "max_density_jump": float(st.max_density_jump)}
         if net is not None:
-            entry["one_step"] = one_step_errors(net, p, parts, dt, K, variants=variants)
+            entry["one_step"] = one_step_errors(net, p, parts, dt, K, variants=variants, smooth=smooth)
         res[f"K{K}"] = entry
         log(f"K={K}: {json.dumps(entry)}")
     return res

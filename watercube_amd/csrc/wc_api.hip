@@ -126,6 +126,7 @@ struct wc_ctx {
     uint32_t* cl_backup_ids = nullptr;
     uint32_t* cl_rows = nullptr;    // row list for feature requests
     uint32_t cl_rows_cap = 0;
+    bool cl_smooth = true;          // average the correction over neighbours in wc_closure_run
 };
 
 struct wc_mlp {
@@ -1176,7 +1177,8 @@ WC_API int wc_closure_update(wc_ctx* c, float dt, const void* corr_dev, float co
     if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "the learned closure runs on single-scene contexts");
     int rc = set_device(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_closure_update(c->b, dt, (const float*)corr_dev, corr_scale, (float*)acc_dev, c->stream));
+    WC_HIP(wc::launch_closure_update(c->b, dt, (const float*)corr_dev, corr_scale, (float*)acc_dev, false,
+                                     c->stream));
     return WC_OK;
 }
 
@@ -1185,6 +1187,12 @@ static int read_rho_max(wc_ctx* c, float* out) {
     WC_HIP(hipMemcpyAsync(&bits, c->b.rho_max, 4, hipMemcpyDeviceToHost, c->stream));
     WC_HIP(hipStreamSynchronize(c->stream));
     memcpy(out, &bits, 4);
+    return WC_OK;
+}
+
+WC_API int wc_closure_set_smoothing(wc_ctx* c, int enable) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "NULL context");
+    c->cl_smooth = enable != 0;
     return WC_OK;
 }
 
@@ -1264,7 +1272,8 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         if (e == hipSuccess) e = wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(net->m.stream);
         if (e == hipSuccess) e = wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream);
-        if (e == hipSuccess) e = wc::launch_closure_update(c->b, dt_large, c->cl_y, g, nullptr, c->stream);
+        if (e == hipSuccess)
+            e = wc::launch_closure_update(c->b, dt_large, c->cl_y, g, nullptr, c->cl_smooth, c->stream);
         if (e != hipSuccess) rc = fail(WC_ERR_HIP, "learned step: %s", hipGetErrorString(e));
         stats->macro_steps += 1;
     }

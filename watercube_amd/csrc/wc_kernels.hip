@@ -547,8 +547,9 @@ __device__ __forceinline__ void integrate_axis(const StepConsts& c, float pf, fl
 struct ForceSums {
     float pfx, pfy, pfz, vfx, vfy, vfz;
 
-    __device__ __forceinline__ void add(const StepConsts& c, const float4& p, const float4& pv, const float4& q,
-                                        const float4& qv) {
+    // returns the pair distance
+    __device__ __forceinline__ float add(const StepConsts& c, const float4& p, const float4& pv, const float4& q,
+                                         const float4& qv) {
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float dist = sqrtf((rx * rx + ry * ry) + rz * rz);
         const float pressure = (pv.w + qv.w) / (2.0f * q.w);
@@ -563,11 +564,13 @@ struct ForceSums {
         vfx = vfx + (c.m * ((qv.x - pv.x) / q.w)) * wv;
         vfy = vfy + (c.m * ((qv.y - pv.y) / q.w)) * wv;
         vfz = vfz + (c.m * ((qv.z - pv.z) / q.w)) * wv;
+        return dist;
     }
 };
 
 // Integration of the learned large step: the reference's update with the network's correction
-// added to the acceleration (far from walls) and the plain acceleration optionally written out.
+// (units of g, times corr_scale) added to the acceleration of particles at least 2h from every
+// wall, and the plain acceleration optionally written out (w = 1 for those particles).
 __device__ __forceinline__ void learned_axis(const StepConsts& c, float pf, float vf, float ext, float denom,
                                              float v0, float x0, float dt, float corr, float& acc_plain, float& vel,
                                              float& pos) {
@@ -586,18 +589,19 @@ __device__ __forceinline__ void learned_axis(const StepConsts& c, float pf, floa
     }
 }
 
+__device__ __forceinline__ bool closure_interior(const StepConsts& c, float x, float y, float z) {
+    const float margin = 2.0f * c.h;
+    return x > margin && y > margin && z > margin && x < c.size - margin && y < c.size - margin &&
+           z < c.size - margin;
+}
+
 __device__ __forceinline__ void learned_integrate(const StepConsts& c, const ForceSums& f, const V3& ext, float denom,
                                                   const float4& p, const float4& pv, uint32_t t, float dt,
-                                                  const float4* __restrict__ corr, float corr_scale,
+                                                  bool have_corr, const V3& kc, float corr_scale,
                                                   float4* __restrict__ acc_out, float4& po, float4& vo) {
     float4 k = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float margin = 2.0f * c.h;
-    const bool interior = p.x > margin && p.y > margin && p.z > margin && p.x < c.size - margin &&
-                          p.y < c.size - margin && p.z < c.size - margin;
-    if (corr && interior) {
-        const float4 q = corr[t];
-        k = make_float4(q.x * corr_scale, q.y * corr_scale, q.z * corr_scale, 0.0f);
-    }
+    const bool interior = closure_interior(c, p.x, p.y, p.z);
+    if (have_corr && interior) k = make_float4(kc.x * corr_scale, kc.y * corr_scale, kc.z * corr_scale, 0.0f);
     float4 a;
     learned_axis(c, f.pfx, f.vfx, ext.x, denom, pv.x, p.x, dt, k.x, a.x, vo.x, po.x);
     learned_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, k.y, a.y, vo.y, po.y);
@@ -608,7 +612,9 @@ __device__ __forceinline__ void learned_integrate(const StepConsts& c, const For
 
 // MODE 0: the reference's force pass.  MODE 1 (learned closure): dt comes from the argument and
 // corr * corr_scale is added to the acceleration of particles at least 2h from every wall; with
-// acc_out the plain acceleration is also written out.
+// acc_out the plain acceleration is also written out.  MODE 2: as MODE 1 with the correction
+// averaged over the particle and its accepted neighbours, weights h - distance (h for the
+// particle itself); neighbours within 2h of a wall count with a zero correction.
 template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __restrict__ posrho_in,
                                                   const float4* __restrict__ velp_in,
@@ -631,6 +637,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
     const float4 pv = velp_in[t];
     const lds_ptr<uint32_t> list = (lds_ptr<uint32_t>)s_list + threadIdx.x;
     ForceSums f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    V3 csum{0.0f, 0.0f, 0.0f};  // MODE 2: weighted sum of neighbour corrections
+    float wsum = 0.0f;
     // neighbour k + 1's records are loaded while neighbour k's terms are formed
     auto drain = [&](int cnt) {
         if (cnt <= 0) return;
@@ -639,7 +647,18 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
         for (int k = 0; k < cnt; ++k) {
             const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
             const float4 qn = posrho_in[un], qvn = velp_in[un];
-            f.add(c, p, pv, q, qv);
+            const float dist = f.add(c, p, pv, q, qv);
+            if constexpr (MODE == 2) {
+                const float w = c.h - dist;
+                if (corr && closure_interior(c, q.x, q.y, q.z)) {
+                    const float4 cj = corr[u];
+                    csum.x += w * cj.x;
+                    csum.y += w * cj.y;
+                    csum.z += w * cj.z;
+                }
+                wsum += w;
+            }
+            u = un;
             q = qn;
             qv = qvn;
         }
@@ -663,7 +682,17 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
         integrate_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, vo.y, po.y);
         integrate_axis(c, f.pfz, f.vfz, ext.z, denom, pv.z, p.z, dt, vo.z, po.z);
     } else {
-        learned_integrate(c, f, ext, denom, p, pv, t, dt_arg, corr, corr_scale, acc_out, po, vo);
+        V3 kc{0.0f, 0.0f, 0.0f};
+        if (corr) {
+            const float4 ct = corr[t];
+            if (MODE == 2) {
+                const float inv = 1.0f / (c.h + wsum);
+                kc = V3{(c.h * ct.x + csum.x) * inv, (c.h * ct.y + csum.y) * inv, (c.h * ct.z + csum.z) * inv};
+            } else {
+                kc = V3{ct.x, ct.y, ct.z};
+            }
+        }
+        learned_integrate(c, f, ext, denom, p, pv, t, dt_arg, corr != nullptr, kc, corr_scale, acc_out, po, vo);
     }
     po.w = p.w;
     vo.w = pv.w;
@@ -711,13 +740,14 @@ __device__ __forceinline__ int drop_farthest(lds_ptr<uint32_t> key, lds_ptr<uint
     return cnt;
 }
 
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
+// two floats -> two bf16 (round to nearest even) in one word, lo in the low half: one
+// v_cvt_pk_bf16_f32 on gfx950
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
+    typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+    const f32x2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
-__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
 
 __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const float4* __restrict__ posrho,
                                                                  const float4* __restrict__ velp,
@@ -863,13 +893,14 @@ hipError_t launch_gather(const DeviceBuffers& b, hipStream_t s) {
     X(256, 32, 4, true)    \
     X(128, 32, 4, true)    \
     X(64, 32, 4, true)     \
-    X(64, 32, 4, false)    \
+    X(64, 32, 8, true)     \
     X(64, 40, 4, false)    \
     X(64, 48, 4, false)    \
     X(64, 64, 4, false)    \
     X(128, 48, 4, false)   \
     X(128, 32, 2, true)    \
-    X(256, 16, 4, true)
+    X(64, 40, 8, false)    \
+    X(128, 32, 8, true)
 
 static const PassVariant kVariants[] = {
 #define WC_ROW(B, C, K, P) {B, C, K, P},
@@ -924,11 +955,16 @@ hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows,
 }
 
 hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
-                                 float* acc_out, hipStream_t s) {
+                                 float* acc_out, bool smooth, hipStream_t s) {
     if (b.sc.blocks64 == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_update<64, 32, 4, true, 1>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc, b.posrho[1],
-                       b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1], b.ids[0], dt,
-                       reinterpret_cast<const float4*>(corr), corr_scale, reinterpret_cast<float4*>(acc_out));
+    if (smooth)
+        hipLaunchKernelGGL((k_update<64, 32, 4, true, 2>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc, b.posrho[1],
+                           b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1], b.ids[0], dt,
+                           reinterpret_cast<const float4*>(corr), corr_scale, reinterpret_cast<float4*>(acc_out));
+    else
+        hipLaunchKernelGGL((k_update<64, 32, 4, true, 1>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc, b.posrho[1],
+                           b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1], b.ids[0], dt,
+                           reinterpret_cast<const float4*>(corr), corr_scale, reinterpret_cast<float4*>(acc_out));
     return hipGetLastError();
 }
 

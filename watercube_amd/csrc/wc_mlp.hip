@@ -30,13 +30,8 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    // round to nearest even; NaN stays NaN
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// float -> bf16, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950; NaN stays NaN)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // SiLU with the hardware reciprocal (about 1 ulp): the network computes in bf16, so the correctly
@@ -193,7 +188,12 @@ __global__ void k_cast(const float* __restrict__ w, uint16_t* __restrict__ rowma
 }
 
 // ------------------------------------------------------------------------------------------
-// Fused inference: Y[n x 4] = MLP(X[n x 208]).  256 threads = 4 waves, 128 rows per workgroup.
+// Fused inference: Y[n x 4] = MLP(X[n x 208]).  Persistent: one workgroup of four waves per CU
+// copies the three hidden layers' weights into LDS once (bf16, transposed [out][in], each row
+// padded by 8 values so that the 32 rows an MFMA operand read touches fall in different banks)
+// and then walks tiles of 128 rows; each wave takes 32 rows of a tile through the four layers,
+// its activations in an LDS buffer of its own.  The output layer's weights (8 KB) are read
+// from global memory.
 // ------------------------------------------------------------------------------------------
 struct InferWeights {
     const uint16_t* w1t;  // [128][208]
@@ -203,66 +203,130 @@ struct InferWeights {
     const float *b1, *b2, *b3, *b4;
 };
 
-template <int KDIM>
-__device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ act, int act_ld, bool act_global,
-                                             int act_rows_left, const uint16_t* __restrict__ wt,
-                                             const float* __restrict__ bias, uint16_t* __restrict__ out_lds, int lane) {
-    f32x16 acc[4] = {{}, {}, {}, {}};
+constexpr int kLd1 = kMlpInPad + 8;   // LDS row length of layer 1's weights
+constexpr int kLdH = kMlpHidden + 8;  // LDS row length of layers 2-3's weights and of activations
+constexpr int kInferWaves = 4;
+constexpr int kInferTile = kInferWaves * 32;  // rows per tile
+constexpr int kInferLdsHalves = kMlpHidden * kLd1 + 2 * kMlpHidden * kLdH + kInferWaves * 32 * kLdH;
+static_assert(kInferLdsHalves * 2 <= 160 * 1024, "weights and activations must fit one CU's LDS");
+
+// LDS written by this wave is visible to this wave's later reads: wait for its LDS operations
+// and keep the compiler from moving memory accesses across.  (Each wave of the inference kernel
+// owns its activation buffer, so no workgroup barrier is needed between layers.)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+constexpr int kKSteps1 = kMlpInPad / 16;  // k steps of layer 1
+
+// Layer 1's A operand for a wave's 32 rows starting at row0 (rows at or past n read as zeros).
+__device__ __forceinline__ void load_rows(uint4 (&a)[kKSteps1], const uint16_t* __restrict__ X, long row0, uint32_t n,
+                                          int lane) {
     const int row = lane & 31, kq = (lane >> 5) * 8;
-#pragma unroll 2
-    for (int k0 = 0; k0 < KDIM; k0 += 16) {
-        uint4 a = make_uint4(0u, 0u, 0u, 0u);
-        if (!act_global || row < act_rows_left) a = *reinterpret_cast<const uint4*>(&act[(long)row * act_ld + k0 + kq]);
+    const bool live = row0 + row < (long)n;
+#pragma unroll
+    for (int s = 0; s < kKSteps1; ++s) a[s] = make_uint4(0u, 0u, 0u, 0u);
+    if (live) {
+        const uint16_t* src = X + (row0 + row) * kMlpInPad + kq;
+#pragma unroll
+        for (int s = 0; s < kKSteps1; ++s) a[s] = *reinterpret_cast<const uint4*>(src + 16 * s);
+    }
+}
+
+// act_out = bf16(silu(A W + b)) for a wave's 32 rows; W^T rows from LDS with stride LDW; A
+// operands from `areg` (KSTEPS registers, layer 1) or from the wave's activation buffer.
+template <int KSTEPS, int LDW, bool FROM_REGS>
+__device__ __forceinline__ void infer_layer(const uint4* areg, const uint16_t* act_in, const uint16_t* __restrict__ wt,
+                                            const float* __restrict__ bias, uint16_t* act_out, int lane) {
+    const int row = lane & 31, kq = (lane >> 5) * 8;
+    // accumulators start at the bias: every element of tile j lies in column j * 32 + row
+    f32x16 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float bj = bias[j * 32 + row];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[j][r] = bj;
+    }
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) {
+        const uint4 av = FROM_REGS ? areg[s] : *reinterpret_cast<const uint4*>(&act_in[row * kLdH + 16 * s + kq]);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const uint4 b = *reinterpret_cast<const uint4*>(&wt[(long)(j * 32 + row) * KDIM + k0 + kq]);
-            acc[j] = mfma(a, b, acc[j]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&wt[(j * 32 + row) * LDW + 16 * s + kq]);
+            acc[j] = mfma(av, bv, acc[j]);
         }
     }
+    // every write below depends on every read above, so act_out may be act_in
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = j * 32 + row;
-        const float bj = bias[col];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            out_lds[rr * kMlpHidden + col] = f2bf(silu(acc[j][r] + bj));
+            act_out[rr * kLdH + col] = f2bf(silu(acc[j][r]));
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ X, InferWeights W,
-                                                   float* __restrict__ Y, uint32_t n) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_h[4][2][32 * kMlpHidden];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long row0 = ((long)blockIdx.x * 4 + wave) * 32;
-    // rows past n read zeros and write nothing; every wave reaches every barrier
-    const int rows_left = (int)max(0L, min((long)32, (long)n - row0));
-    uint16_t* h0 = s_h[wave][0];
-    uint16_t* h1 = s_h[wave][1];
-    hidden_layer<kMlpInPad>(X + (rows_left ? row0 : 0) * kMlpInPad, kMlpInPad, true, rows_left, W.w1t, W.b1, h0, lane);
-    __syncthreads();
-    hidden_layer<kMlpHidden>(h0, kMlpHidden, false, 32, W.w2t, W.b2, h1, lane);
-    __syncthreads();
-    hidden_layer<kMlpHidden>(h1, kMlpHidden, false, 32, W.w3t, W.b3, h0, lane);
-    __syncthreads();
-    // output layer: one 32x32 tile, columns 0..2 used
-    f32x16 acc = {};
-    const int row = lane & 31, kq = (lane >> 5) * 8;
-#pragma unroll
-    for (int k0 = 0; k0 < kMlpHidden; k0 += 16) {
-        const uint4 a = *reinterpret_cast<const uint4*>(&h0[row * kMlpHidden + k0 + kq]);
-        const uint4 b = *reinterpret_cast<const uint4*>(&W.w4t[(long)row * kMlpHidden + k0 + kq]);
-        acc = mfma(a, b, acc);
+__global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(const uint16_t* __restrict__ X, InferWeights W,
+                                                                 float* __restrict__ Y, uint32_t n) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_mem[kInferLdsHalves];
+    uint16_t* w1 = s_mem;                          // [128][kLd1]
+    uint16_t* w2 = w1 + kMlpHidden * kLd1;         // [128][kLdH]
+    uint16_t* w3 = w2 + kMlpHidden * kLdH;         // [128][kLdH]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint16_t* act = w3 + kMlpHidden * kLdH + wave * 32 * kLdH;  // this wave's [32][kLdH]
+
+    const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
+    uint4 xa[kKSteps1];
+    if (blockIdx.x < tiles) load_rows(xa, X, (long)blockIdx.x * kInferTile + wave * 32, n, lane);
+
+    // weights into LDS, 8 values per copy
+    for (int i = tid; i < kMlpHidden * (kMlpInPad / 8); i += kInferWaves * 64) {
+        const int r = i / (kMlpInPad / 8), q = i % (kMlpInPad / 8);
+        *reinterpret_cast<uint4*>(&w1[r * kLd1 + q * 8]) = reinterpret_cast<const uint4*>(W.w1t)[i];
     }
-    const int col = row;  // output column of this lane
-    if (col < 4) {
-        const float bc = col < kMlpOut ? W.b4[col] : 0.0f;
+    for (int i = tid; i < kMlpHidden * (kMlpHidden / 8); i += kInferWaves * 64) {
+        const int r = i / (kMlpHidden / 8), q = i % (kMlpHidden / 8);
+        *reinterpret_cast<uint4*>(&w2[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w2t)[i];
+        *reinterpret_cast<uint4*>(&w3[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w3t)[i];
+    }
+    __syncthreads();
+
+    // From here on the waves are independent: each walks the workgroup's tiles with its own
+    // activation buffer; the next tile's input rows load while the current tile is computed.
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const long row0 = (long)tile * kInferTile + wave * 32;
+        const int rows_left = (int)max(0L, min(32L, (long)n - row0));
+        infer_layer<kKSteps1, kLd1, true>(xa, nullptr, w1, W.b1, act, lane);
+        const uint32_t next = tile + gridDim.x;
+        if (next < tiles) load_rows(xa, X, (long)next * kInferTile + wave * 32, n, lane);
+        wave_lds_sync();
+        infer_layer<kMlpHidden / 16, kLdH, false>(nullptr, act, w2, W.b2, act, lane);
+        wave_lds_sync();
+        infer_layer<kMlpHidden / 16, kLdH, false>(nullptr, act, w3, W.b3, act, lane);
+        wave_lds_sync();
+        // output layer: one 32x32 tile, columns 0..2 used
+        f32x16 acc = {};
+        const int row = lane & 31, kq = (lane >> 5) * 8;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (rr < rows_left) Y[(row0 + rr) * 4 + col] = col < kMlpOut ? acc[r] + bc : 0.0f;
+        for (int k0 = 0; k0 < kMlpHidden; k0 += 16) {
+            const uint4 av = *reinterpret_cast<const uint4*>(&act[row * kLdH + k0 + kq]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&W.w4t[(long)row * kMlpHidden + k0 + kq]);
+            acc = mfma(av, bv, acc);
         }
+        const int col = row;  // output column of this lane
+        if (col < 4) {
+            const float bc = col < kMlpOut ? W.b4[col] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (rr < rows_left) Y[(row0 + rr) * 4 + col] = col < kMlpOut ? acc[r] + bc : 0.0f;
+            }
+        }
+        wave_lds_sync();  // the next tile's first layer overwrites act
     }
 }
 
@@ -331,7 +395,9 @@ hipError_t mlp_infer(const Mlp& m, const uint16_t* X, float* Y, uint32_t n, hipS
     if (n == 0) return hipSuccess;
     const MlpLayout L = mlp_layout();
     InferWeights W{m.w1t, m.w2t, m.w3t, m.w4t, m.master + L.b1, m.master + L.b2, m.master + L.b3, m.master + L.b4};
-    hipLaunchKernelGGL(k_mlp_infer, dim3(nblk(n, 128)), dim3(256), 0, s, X, W, Y, n);
+    // one workgroup per CU (the kernel holds most of a CU's LDS), each walking several tiles
+    const unsigned tiles = nblk(n, kInferTile);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(tiles < 256u ? tiles : 256u), dim3(kInferWaves * 64), 0, s, X, W, Y, n);
     return hipGetLastError();
 }
 
