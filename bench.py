@@ -10,7 +10,13 @@ Besides the throughput the line carries
   roofline     the dominant kernel's algorithmic bytes per launch over its average duration,
                measured with HIP events on the context's stream (eager launches, same state),
                against the 8 TB/s HBM3E peak; `traffic` is the HBM bytes per launch that the
-               committed rocprofv3 FETCH_SIZE / WRITE_SIZE profile of that kernel measured;
+               committed rocprofv3 FETCH_SIZE / WRITE_SIZE profile of that kernel measured, and
+               `valu` its VALU issue rate (SQ_INSTS_VALU of the committed profile over the live
+               duration) against one wave64 instruction per CU per cycle;
+  batched      256 independent scenes of 4,096..262,144 particles stepped together;
+  scale        the 17,777,216-particle scene;
+  learned      plain and learned steps of K*dt against plain steps of dt (density errors,
+               simulated time per second), and the closure network's matrix-core utilisation;
   cpu_baseline the CPU restatement of the same step on this host's cores, on a bounded sample.
 """
 from __future__ import annotations
@@ -135,6 +141,28 @@ def batched_section(steps: int, warmup: int = 10, latency_samples: int = 20) -> 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
+def scale_section(steps: int, warmup: int = 5) -> dict:
+    """The 17,777,216-particle scene (256^3 block plus a falling 1M ball, R = 250): graph-launched
+    steps per second and the per-pass durations, with the variants the autotuner picks there."""
+    from watercube_amd import Fluid, scenes
+
+    p, parts, dt, _ = scenes.workload("block_sphere_17m")
+    n = int(p.num_particles)
+    with Fluid(p, parts) as f:
+        tuning = f.tune(dt, reps=2)
+        f.step(dt, warmup)
+        f.synchronize()
+        t0 = time.perf_counter()
+        f.step(dt, steps)
+        f.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = f.profile(dt, 5)
+    return {"workload": "block_sphere_17m", "particles": n, "grid_res": int(p.grid_res), "steps": steps,
+            "value": n * steps / elapsed, "unit": "particle-steps/s", "ms_per_step": elapsed * 1e3 / steps,
+            "pass_ms": {k: v / 5 for k, v in prof.items()},
+            "tuning": [tuning["variants"][tuning["density_variant"]], tuning["variants"][tuning["update_variant"]]]}
+
+
 def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int = 20) -> dict:
     """Large steps on the 1,000,000-particle block.  Over `horizon` small steps of dt from the same
     state: plain steps of dt (the reference run), plain steps of K*dt, and judged learned steps
@@ -213,6 +241,7 @@ def main() -> int:
     ap.add_argument("--weights", default=str(ROOT / "watercube_amd" / "weights" / "closure.npy"),
                     help="closure network weights for the learned section (skipped if absent)")
     ap.add_argument("--learned-horizon", type=int, default=100)
+    ap.add_argument("--scale-steps", type=int, default=50, help="steps of the 17.8M-particle scene (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step runs on a single MI355X)",
@@ -297,6 +326,9 @@ def main() -> int:
     if args.batch_steps > 0:
         out["batched"] = batched_section(args.batch_steps)
         print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
+    if args.scale_steps > 0:
+        out["scale"] = scale_section(args.scale_steps)
+        print(f"# scale: {out['scale']}", file=sys.stderr, flush=True)
     weights = Path(args.weights)
     if weights.exists():
         out["learned"] = learned_section(p, parts, dt, weights, horizon=args.learned_horizon)
