@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 output for the SPH step's kernels.
 
-    python3 -m watercube_amd.profile_summary --trace DIR --fetch DIR --write DIR --sq DIR \
+    python3 -m watercube_amd.profile_summary --trace DIR [--fetch DIR --write DIR] [--sq DIR] \
         --workload block_1m --command "..." --out profiles/
 
 Reads the CSV files rocprofv3 writes (``*_kernel_stats.csv`` of a ``--kernel-trace --stats`` run,
@@ -53,10 +53,22 @@ def counters(pmc_dir: Path) -> dict:
     return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
 
 
-def summarise(trace: Path, fetch: Path, write: Path, sq: Path | None, workload: str, particles: int,
-              command: str, out: Path) -> dict:
+def summarise(trace: Path, fetch: Path | None, write: Path | None, sq: Path | None, workload: str, particles: int,
+              command: str, out: Path, kernels=STEP_KERNELS) -> dict:
     ks = kernel_stats(trace)
-    f, w = counters(fetch), counters(write)
+    out.mkdir(parents=True, exist_ok=True)
+    traffic = {"workload": workload, "particles": particles, "command": command, "kernels": {}}
+    if fetch is not None and write is not None:
+        traffic = _traffic(ks, counters(fetch), counters(write), workload, particles, command, out)
+    _kernel_table(ks, workload, particles, command, out)
+    if sq is not None:
+        _sq_table(ks, counters(sq), traffic, workload, particles, command, out, kernels)
+        if fetch is not None and write is not None:
+            (out / f"traffic_{workload}.json").write_text(json.dumps(traffic, indent=1) + "\n")
+    return traffic
+
+
+def _traffic(ks, f, w, workload, particles, command, out) -> dict:
     copy_bytes = 32.0 * particles
     f_factor = copy_bytes / (f["k_split"]["FETCH_SIZE"] * 1024.0)
     w_factor = copy_bytes / (w["k_split"]["WRITE_SIZE"] * 1024.0)
@@ -80,10 +92,12 @@ def summarise(trace: Path, fetch: Path, write: Path, sq: Path | None, workload: 
                                  "read_bytes": rd, "write_bytes": wr, "hbm_bytes_per_launch": rd + wr}
         lines.append(f"| {k} | {f[k]['FETCH_SIZE']:.0f} | {w[k]['WRITE_SIZE']:.0f} | {rd / 1e6:.2f} | "
                      f"{wr / 1e6:.2f} | {(rd + wr) / 1e6:.2f} |")
-    out.mkdir(parents=True, exist_ok=True)
     (out / f"traffic_{workload}.json").write_text(json.dumps(traffic, indent=1) + "\n")
     (out / f"traffic_{workload}.md").write_text("\n".join(lines) + "\n")
+    return traffic
 
+
+def _kernel_table(ks, workload, particles, command, out) -> None:
     lines = [f"# Kernel durations, {workload} ({particles} particles)", "", f"Command: `{command}`", "",
              "| kernel | calls | average us | min us | max us | % of GPU time |", "|---|---|---|---|---|---|"]
     for k, v in sorted(ks.items(), key=lambda kv: -kv[1]["percent"]):
@@ -91,41 +105,41 @@ def summarise(trace: Path, fetch: Path, write: Path, sq: Path | None, workload: 
                      f"{v['max_ns'] / 1e3:.2f} | {v['percent']:.2f} |")
     (out / f"kernels_{workload}.md").write_text("\n".join(lines) + "\n")
 
-    if sq is not None:
-        s = counters(sq)
-        lines = [f"# SQ instruction counters per launch, {workload} ({particles} particles)", "",
-                 f"Command: `{command}`", "",
-                 "Chip totals per dispatch.  VALU utilisation = SQ_INSTS_VALU / (duration x 256 CUs x 2.4 GHz), "
-                 "duration from the kernel trace above.", "",
-                 "| kernel | waves | VALU | SALU | VMEM rd | LDS | VALU per wave | VALU utilisation |",
-                 "|---|---|---|---|---|---|---|---|"]
-        for k in STEP_KERNELS:
-            if k not in s or k not in ks:
-                continue
-            c = s[k]
-            util = c["SQ_INSTS_VALU"] / (ks[k]["avg_ns"] * 1e-9 * VALU_WAVE_INSTR_PER_S)
-            traffic["kernels"].setdefault(k, {})["valu_wave_instr"] = c["SQ_INSTS_VALU"]
-            traffic["kernels"][k]["valu_utilisation"] = util
-            lines.append(f"| {k} | {c['SQ_WAVES']:.0f} | {c['SQ_INSTS_VALU']:.4g} | {c['SQ_INSTS_SALU']:.4g} | "
-                         f"{c['SQ_INSTS_VMEM_RD']:.4g} | {c['SQ_INSTS_LDS']:.4g} | "
-                         f"{c['SQ_INSTS_VALU'] / max(c['SQ_WAVES'], 1):.0f} | {util:.2f} |")
-        (out / f"sq_{workload}.md").write_text("\n".join(lines) + "\n")
-        (out / f"traffic_{workload}.json").write_text(json.dumps(traffic, indent=1) + "\n")
-    return traffic
+
+def _sq_table(ks, s, traffic, workload, particles, command, out, kernels) -> None:
+    lines = [f"# SQ instruction counters per launch, {workload} ({particles} particles)", "",
+             f"Command: `{command}`", "",
+             "Chip totals per dispatch.  VALU utilisation = SQ_INSTS_VALU / (duration x 256 CUs x 2.4 GHz), "
+             "duration from the kernel trace above.", "",
+             "| kernel | waves | VALU | SALU | VMEM rd | LDS | VALU per wave | VALU utilisation |",
+             "|---|---|---|---|---|---|---|---|"]
+    for k in kernels:
+        if k not in s or k not in ks:
+            continue
+        c = s[k]
+        util = c["SQ_INSTS_VALU"] / (ks[k]["avg_ns"] * 1e-9 * VALU_WAVE_INSTR_PER_S)
+        traffic["kernels"].setdefault(k, {})["valu_wave_instr"] = c["SQ_INSTS_VALU"]
+        traffic["kernels"][k]["valu_utilisation"] = util
+        lines.append(f"| {k} | {c['SQ_WAVES']:.0f} | {c['SQ_INSTS_VALU']:.4g} | {c['SQ_INSTS_SALU']:.4g} | "
+                     f"{c['SQ_INSTS_VMEM_RD']:.4g} | {c['SQ_INSTS_LDS']:.4g} | "
+                     f"{c['SQ_INSTS_VALU'] / max(c['SQ_WAVES'], 1):.0f} | {util:.2f} |")
+    (out / f"sq_{workload}.md").write_text("\n".join(lines) + "\n")
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True, type=Path)
-    ap.add_argument("--fetch", required=True, type=Path)
-    ap.add_argument("--write", required=True, type=Path)
+    ap.add_argument("--fetch", type=Path, default=None)
+    ap.add_argument("--write", type=Path, default=None)
+    ap.add_argument("--kernels", default=",".join(STEP_KERNELS), help="kernels of the SQ table")
     ap.add_argument("--sq", type=Path, default=None)
     ap.add_argument("--workload", default="block_1m")
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--command", required=True)
     ap.add_argument("--out", type=Path, default=Path("profiles"))
     a = ap.parse_args(argv)
-    t = summarise(a.trace, a.fetch, a.write, a.sq, a.workload, a.particles, a.command, a.out)
+    t = summarise(a.trace, a.fetch, a.write, a.sq, a.workload, a.particles, a.command, a.out,
+                  kernels=tuple(a.kernels.split(",")))
     print(json.dumps({k: round(v.get("hbm_bytes_per_launch", 0) / 1e6, 2) for k, v in t["kernels"].items()}))
     return 0
 
