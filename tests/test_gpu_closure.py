@@ -192,3 +192,36 @@ def test_identities_follow_particles():
         f.sort_.run()
         order = f.sort_.getSortedBuffer()
         assert np.array_equal(f.ids(2), ids[order])
+
+
+def test_neighbour_averaged_correction_of_a_constant_field():
+    """A network whose output is the constant c: averaged over neighbours, the correction of a
+    particle whose neighbours all lie 2h or more from the walls is c again (up to rounding), while
+    within 3h of a wall the neighbours there count with zero and pull it down."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    K = 2
+    Lo = MO.layout()
+    w = np.zeros(ClosureNet.num_params(), np.float32)
+    c = np.array([0.5, -0.3, 0.2], np.float32)
+    w[Lo["b4"][0]:Lo["b4"][0] + 3] = c
+    net = ClosureNet(max_batch=64)
+    net.set_params(w)
+    out = {}
+    for smooth in (0, 1):
+        with Fluid(p, parts) as f:
+            L.check(L.lib().wc_closure_set_smoothing(f.ctx, smooth), "smoothing")
+            st = L.wc_closure_stats()
+            L.check(L.lib().wc_closure_run(f.ctx, net.h, C.c_float(dt), K, 1, C.c_float(1e30), C.byref(st)),
+                    "wc_closure_run")
+            out[smooth] = f.particles(1)
+    h = float(p.kernel_radius)
+    # both runs sorted the same initial state, so slot i holds the same particle in both
+    v_own, v_avg = out[0]["velocity"], out[1]["velocity"]
+    x = out[0]["position"]
+    dist_wall = np.minimum(x, 1.0 - x).min(axis=1)
+    scale = float(np.abs(c).max()) * 9.81 * K * dt
+    deep = dist_wall > 3.2 * h
+    band = (dist_wall > 2.2 * h) & (dist_wall < 2.8 * h)
+    assert deep.sum() > 100 and band.sum() > 100
+    assert np.abs(v_avg[deep] - v_own[deep]).max() < 1e-3 * scale
+    assert np.abs(v_avg[band] - v_own[band]).max() > 0.05 * scale
