@@ -134,7 +134,7 @@ def batched_section(steps: int, warmup: int = 10, latency_samples: int = 20) -> 
         "ms_per_step": elapsed * 1e3 / steps,
         "scene_latency_ms_p50": float(np.percentile(lat, 50)),
         "scene_latency_ms_p99": float(np.percentile(lat, 99)),
-        "tuning": [tuning["density_variant"], tuning["update_variant"]],
+        "tuning": [tuning["density_variant"], tuning["update_variant"], tuning["neighbour_list"]],
     }
 
 
@@ -160,7 +160,8 @@ def scale_section(steps: int, warmup: int = 5) -> dict:
     return {"workload": "block_sphere_17m", "particles": n, "grid_res": int(p.grid_res), "steps": steps,
             "value": n * steps / elapsed, "unit": "particle-steps/s", "ms_per_step": elapsed * 1e3 / steps,
             "pass_ms": {k: v / 5 for k, v in prof.items()},
-            "tuning": [tuning["variants"][tuning["density_variant"]], tuning["variants"][tuning["update_variant"]]]}
+            "tuning": [tuning["variants"][tuning["density_variant"]], tuning["variants"][tuning["update_variant"]],
+                       tuning["neighbour_list"]]}
 
 
 def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int = 20) -> dict:
@@ -259,9 +260,12 @@ def main() -> int:
     with Fluid(p, parts) as f:
         tuning = f.tune(dt, reps=5)  # untimed: picks the fastest variant of each neighbour pass
         print(f"# tuned variants: density {tuning['variants'][tuning['density_variant']]} "
-              f"update {tuning['variants'][tuning['update_variant']]}", file=sys.stderr, flush=True)
+              f"update {tuning['variants'][tuning['update_variant']]} neighbour lists {tuning['neighbour_list']}",
+              file=sys.stderr, flush=True)
         print("# variant ms density: " + " ".join(f"{x:.4f}" for x in tuning["density_ms"]) +
               " | update: " + " ".join(f"{x:.4f}" for x in tuning["update_ms"]), file=sys.stderr, flush=True)
+        print("# with lists, density: " + " ".join(f"{x:.4f}" for x in tuning["density_list_ms"]) +
+              " | update: " + " ".join(f"{x:.4f}" for x in tuning["update_list_ms"]), file=sys.stderr, flush=True)
         f.step(dt, args.warmup)
         f.synchronize()
         t0 = time.perf_counter()
@@ -307,7 +311,8 @@ def main() -> int:
         "config": {"workload": WORKLOAD, "particles": n, "grid_res": int(p.grid_res), "dt": dt,
                    "method": "plain", "kernel_radius": float(p.kernel_radius),
                    "tuning": {"density": tuning["variants"][tuning["density_variant"]],
-                              "update": tuning["variants"][tuning["update_variant"]]}},
+                              "update": tuning["variants"][tuning["update_variant"]],
+                              "neighbour_list": tuning["neighbour_list"]}},
         "simulated_s_per_s": dt * args.steps / elapsed,
         "pass_ms": per_launch_ms,
         "roofline": {

@@ -237,6 +237,10 @@ WC_API const char* wc_pass_name(int k);
 typedef struct wc_tuning {
     int32_t density_variant;
     int32_t update_variant;
+    /* 1: the density pass also stores each particle's accepted neighbours (up to 64, in
+     * enumeration order) and the force pass reads them instead of searching again; a particle
+     * with more neighbours is searched for as with 0.  Same results either way. */
+    int32_t neighbour_list;
 } wc_tuning;
 WC_API int wc_num_variants(void);
 WC_API int wc_variant_info(int v, int32_t* block, int32_t* list_cap, int32_t* chunk, int32_t* plane_drain);
@@ -244,11 +248,14 @@ WC_API int wc_get_tuning(const wc_ctx* ctx, wc_tuning* out);
 WC_API int wc_set_tuning(wc_ctx* ctx, const wc_tuning* tuning);
 
 /* Autotuner: sorts the current state once, then times every variant of the density pass and of
- * the force pass (reps launches each, HIP events on the context's stream) and keeps the fastest
- * of each for later steps.  dt > 0 sets every scene's step size first; dt <= 0 keeps them.  Particle buffer 1 is restored afterwards; buffer 2 (the sorted copy
- * the next step rebuilds anyway) is left sorted.  ms_density / ms_update, when not NULL, receive
- * wc_num_variants() average launch times in milliseconds. */
-WC_API int wc_tune(wc_ctx* ctx, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update);
+ * the force pass (reps launches each, HIP events on the context's stream), both without and with
+ * neighbour lists, and keeps the fastest variant pair of the faster mode for later steps.  dt > 0
+ * sets every scene's step size first; dt <= 0 keeps them.  Particle buffer 1 is restored
+ * afterwards; buffer 2 (the sorted copy the next step rebuilds anyway) is left sorted.  Each
+ * non-NULL array receives wc_num_variants() average launch times in milliseconds: the two passes
+ * without lists (ms_density, ms_update) and with lists (ms_density_list, ms_update_list). */
+WC_API int wc_tune(wc_ctx* ctx, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update,
+                   double* ms_density_list, double* ms_update_list);
 
 /* ------------------------------------------------------------------------------------------
  * Learned closure.  No reference counterpart: the reference has a single explicit step

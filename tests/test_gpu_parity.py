@@ -115,12 +115,13 @@ def test_every_tuning_variant_is_bit_identical(oracle):
     st = oracle.OracleState(p, parts)
     st.step(dt, 5)
     nv = L.lib().wc_num_variants()
-    for v in range(nv):
-        _log(f"variant {v}: {L.variants()[v]}")
-        with Fluid(p, parts) as f:
-            f.set_tuning(v, v)
-            f.step(dt, 5)
-            _check_records(f.particles(1), st.buf1, f"variant {v} {L.variants()[v]}")
+    for lists in (0, 1):
+        for v in range(nv):
+            _log(f"variant {v}: {L.variants()[v]}, neighbour lists {lists}")
+            with Fluid(p, parts) as f:
+                f.set_tuning(v, v, lists)
+                f.step(dt, 5)
+                _check_records(f.particles(1), st.buf1, f"variant {v} {L.variants()[v]} lists {lists}")
 
 
 def test_autotuner_keeps_state_and_results(oracle):
@@ -134,6 +135,8 @@ def test_autotuner_keeps_state_and_results(oracle):
         _log(f"tuned: {info['density_variant']} {info['update_variant']}")
         assert f.particles(1).tobytes() == before.tobytes()
         assert all(x > 0 for x in info["density_ms"]) and all(x > 0 for x in info["update_ms"])
+        assert all(x > 0 for x in info["density_list_ms"]) and all(x > 0 for x in info["update_list_ms"])
+        assert f.get_tuning() == (info["density_variant"], info["update_variant"], info["neighbour_list"])
         f.step(dt, 2)
         _check_records(f.particles(1), st.buf1, "after tuning")
 
@@ -191,6 +194,13 @@ def test_dense_cluster_uses_fallback_path(oracle):
     rng = np.random.default_rng(9)
     parts = _records(rng.uniform(0.5, 0.5 + h / 4, (400, 3)).astype(np.float32))
     _trajectory_case(p, parts, 2e-5, 3, oracle, "dense cluster")
+    # with neighbour lists: every list overflows (399 > 64) and each particle searches again
+    st = oracle.OracleState(p, parts)
+    st.step(2e-5, 3)
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, 1)
+        f.step(2e-5, 3)
+        _check_records(f.particles(1), st.buf1, "dense cluster, neighbour lists")
 
 
 def test_mouse_force_trajectory(oracle):

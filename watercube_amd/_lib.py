@@ -103,7 +103,7 @@ class wc_closure_stats(C.Structure):
 
 
 class wc_tuning(C.Structure):
-    _fields_ = [("density_variant", C.c_int32), ("update_variant", C.c_int32)]
+    _fields_ = [("density_variant", C.c_int32), ("update_variant", C.c_int32), ("neighbour_list", C.c_int32)]
 
 
 _P = C.POINTER
@@ -148,7 +148,8 @@ SIGNATURES = {
     "wc_variant_info": (C.c_int, [C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),
     "wc_get_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
     "wc_set_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
-    "wc_tune": (C.c_int, [_vp, C.c_float, C.c_uint32, _P(wc_tuning), _P(C.c_double), _P(C.c_double)]),
+    "wc_tune": (C.c_int, [_vp, C.c_float, C.c_uint32, _P(wc_tuning), _P(C.c_double), _P(C.c_double),
+                          _P(C.c_double), _P(C.c_double)]),
     "wc_mlp_create": (C.c_int, [C.c_uint32, C.c_uint64, _P(_vp)]),
     "wc_mlp_destroy": (C.c_int, [_vp]),
     "wc_mlp_num_params": (C.c_int, []),

@@ -107,11 +107,12 @@ class SceneBatch:
 
     def tune(self, reps: int = 3) -> dict:
         nv = L.lib().wc_num_variants()
-        md, mu = (C.c_double * nv)(), (C.c_double * nv)()
+        md, mu, mdl, mul = ((C.c_double * nv)() for _ in range(4))
         t = L.wc_tuning()
-        L.check(L.lib().wc_tune(self.ctx, C.c_float(0.0), reps, C.byref(t), md, mu), "wc_tune")
+        L.check(L.lib().wc_tune(self.ctx, C.c_float(0.0), reps, C.byref(t), md, mu, mdl, mul), "wc_tune")
         return {"density_variant": t.density_variant, "update_variant": t.update_variant,
-                "density_ms": list(md), "update_ms": list(mu)}
+                "neighbour_list": t.neighbour_list, "density_ms": list(md), "update_ms": list(mu),
+                "density_list_ms": list(mdl), "update_list_ms": list(mul)}
 
     def scene_particles(self, s: int, which: int = 1) -> np.ndarray:
         n = int(self.params[s].num_particles)

@@ -371,7 +371,7 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
     rho0 = float(p.rest_density)
     with Fluid(p, parts) as f:
         tuning = f.tune(dt, reps=2)
-    variants = (tuning["density_variant"], tuning["update_variant"])
+    variants = (tuning["density_variant"], tuning["update_variant"], tuning["neighbour_list"])
 
     def fluid():
         f = Fluid(p, parts).setup()

@@ -118,6 +118,7 @@ struct wc_ctx {
     bool graph_valid = false;
     int density_variant = 2;  // (64 threads, 32 list entries, 4 in flight, per-plane drain) until tuned
     int update_variant = 2;
+    bool neighbour_list = false;  // density writes neighbour lists, the force pass reads them
     double clock_khz = 100000.0;  // wall_clock64 rate
     // learned-closure workspace (allocated on first use)
     uint16_t* cl_x = nullptr;       // features, n x 208 bf16
@@ -207,7 +208,7 @@ void free_ctx_buffers(wc_ctx* c) {
                     b.slot_tmp,    b.sorted,      b.skey,        b.counts,      b.range,     b.scan_status,
                     b.scan_ticket, b.error_flag,  b.scene_done,  b.step_start,  c->staging,  c->d_desc,
                     c->d_tables,   b.rho_max,     b.ids[0],      b.ids[1],      c->cl_x,     c->cl_y,
-                    c->cl_backup[0], c->cl_backup[1], c->cl_backup_ids, c->cl_rows};
+                    c->cl_backup[0], c->cl_backup[1], c->cl_backup_ids, c->cl_rows, b.nlist, b.ncount};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     memset(&b, 0, sizeof(b));
@@ -239,6 +240,21 @@ void destroy_ctx(wc_ctx* c) {
     delete c;
 }
 
+// Neighbour-list storage, allocated the first time the list mode is used.
+int ensure_neighbour_lists(wc_ctx* c) {
+    if (c->b.nlist && c->b.ncount) return WC_OK;
+    const size_t n = c->particle_cap ? c->particle_cap : 1;
+    if (!c->b.nlist && hipMalloc(&c->b.nlist, (size_t)wc::kNeighbourListMax * n * 4) != hipSuccess) {
+        c->b.nlist = nullptr;
+        return fail(WC_ERR_NO_MEMORY, "neighbour lists for %zu particles", n);
+    }
+    if (!c->b.ncount && hipMalloc(&c->b.ncount, n * 4) != hipSuccess) {
+        c->b.ncount = nullptr;
+        return fail(WC_ERR_NO_MEMORY, "neighbour counts for %zu particles", n);
+    }
+    return WC_OK;
+}
+
 int enqueue_sort(wc_ctx* c) {
     WC_HIP(wc::launch_hash(c->b, c->stream));
     WC_HIP(wc::launch_scan(c->b, c->stream));
@@ -250,8 +266,8 @@ int enqueue_sort(wc_ctx* c) {
 int enqueue_step(wc_ctx* c) {
     int rc = enqueue_sort(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_density(c->b, c->density_variant, c->stream));
-    WC_HIP(wc::launch_update(c->b, c->update_variant, c->stream));
+    WC_HIP(wc::launch_density(c->b, c->density_variant, c->neighbour_list, c->stream));
+    WC_HIP(wc::launch_update(c->b, c->update_variant, c->neighbour_list, c->stream));
     return WC_OK;
 }
 
@@ -633,7 +649,7 @@ WC_API int wc_density(wc_ctx* c) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     int rc = set_device(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_density(c->b, c->density_variant, c->stream));
+    WC_HIP(wc::launch_density(c->b, c->density_variant, c->neighbour_list, c->stream));
     return WC_OK;
 }
 
@@ -643,7 +659,7 @@ WC_API int wc_update(wc_ctx* c, float dt) {
     if (rc) return rc;
     rc = set_device(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_update(c->b, c->update_variant, c->stream));
+    WC_HIP(wc::launch_update(c->b, c->update_variant, c->neighbour_list, c->stream));
     return WC_OK;
 }
 
@@ -749,9 +765,9 @@ WC_API int wc_profile_steps(wc_ctx* c, float dt, uint32_t nsteps, double ms_tota
         if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
         if (err == hipSuccess) err = wc::launch_gather(c->b, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[4], c->stream);
-        if (err == hipSuccess) err = wc::launch_density(c->b, c->density_variant, c->stream);
+        if (err == hipSuccess) err = wc::launch_density(c->b, c->density_variant, c->neighbour_list, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[5], c->stream);
-        if (err == hipSuccess) err = wc::launch_update(c->b, c->update_variant, c->stream);
+        if (err == hipSuccess) err = wc::launch_update(c->b, c->update_variant, c->neighbour_list, c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[6], c->stream);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
@@ -786,6 +802,7 @@ WC_API int wc_get_tuning(const wc_ctx* c, wc_tuning* out) {
     if (!c || !out) return fail(WC_ERR_INVALID_ARG, "NULL argument");
     out->density_variant = c->density_variant;
     out->update_variant = c->update_variant;
+    out->neighbour_list = c->neighbour_list ? 1 : 0;
     return WC_OK;
 }
 
@@ -794,13 +811,21 @@ WC_API int wc_set_tuning(wc_ctx* c, const wc_tuning* t) {
     const int nv = wc::num_variants();
     if (t->density_variant < 0 || t->density_variant >= nv || t->update_variant < 0 || t->update_variant >= nv)
         return fail(WC_ERR_INVALID_ARG, "variant outside [0, %d)", nv);
+    if (t->neighbour_list) {
+        const int rc = set_device(c);
+        if (rc) return rc;
+        const int ra = ensure_neighbour_lists(c);
+        if (ra) return ra;
+    }
     c->density_variant = t->density_variant;
     c->update_variant = t->update_variant;
+    c->neighbour_list = t->neighbour_list != 0;
     drop_graph(c);
     return WC_OK;
 }
 
-WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update) {
+WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update,
+                   double* ms_density_list, double* ms_update_list) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     if (reps == 0) reps = 1;
     int rc = dt > 0.0f ? set_all_dt(c, &dt, true) : WC_OK;
@@ -809,8 +834,13 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
     if (rc) return rc;
     const int nv = wc::num_variants();
     const size_t n = c->total_particles;
-    std::vector<double> td(nv, 0.0), tu(nv, 0.0);
+    // [mode][pass][variant]: mode 0 searches in both passes, mode 1 uses neighbour lists
+    std::vector<double> t[2][2];
+    for (auto& m : t)
+        for (auto& v : m) v.assign(nv, 0.0);
     if (n > 0) {
+        rc = ensure_neighbour_lists(c);
+        if (rc) return rc;
         // Keep buffer 1 (the state the caller sees); the passes below rewrite it.
         float4 *save_pos = nullptr, *save_vel = nullptr;
         WC_HIP(hipMalloc(&save_pos, n * sizeof(float4)));
@@ -826,21 +856,25 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
         if (err == hipSuccess)
             err = hipMemcpyAsync(save_vel, c->b.velp[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
         if (err == hipSuccess) err = (enqueue_sort(c) == WC_OK) ? hipSuccess : hipErrorUnknown;
-        // Both passes are idempotent on the sorted buffer: density rewrites the same densities,
-        // the force pass rewrites buffer 1 from buffer 2 alone.
-        for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
-            for (int v = 0; v < nv && err == hipSuccess; ++v) {
-                auto launch = [&]() {
-                    return pass == 0 ? wc::launch_density(c->b, v, c->stream) : wc::launch_update(c->b, v, c->stream);
-                };
-                err = launch();  // warm-up
-                if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
-                for (uint32_t r = 0; r < reps && err == hipSuccess; ++r) err = launch();
-                if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
-                if (err == hipSuccess) err = hipEventSynchronize(e1);
-                float ms = 0.0f;
-                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-                (pass == 0 ? td : tu)[v] = (double)ms / reps;
+        // Both passes are idempotent on the sorted buffer: density rewrites the same densities
+        // (and, with lists, the same lists), the force pass rewrites buffer 1 from buffer 2 alone.
+        // In list mode the density variants run first, so every force-pass variant reads lists.
+        for (int mode = 0; mode < 2 && err == hipSuccess; ++mode) {
+            for (int pass = 0; pass < 2 && err == hipSuccess; ++pass) {
+                for (int v = 0; v < nv && err == hipSuccess; ++v) {
+                    auto launch = [&]() {
+                        return pass == 0 ? wc::launch_density(c->b, v, mode == 1, c->stream)
+                                         : wc::launch_update(c->b, v, mode == 1, c->stream);
+                    };
+                    err = launch();  // warm-up
+                    if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
+                    for (uint32_t r = 0; r < reps && err == hipSuccess; ++r) err = launch();
+                    if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+                    if (err == hipSuccess) err = hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                    t[mode][pass][v] = (double)ms / reps;
+                }
             }
         }
         if (err == hipSuccess)
@@ -853,22 +887,29 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
         (void)hipFree(save_pos);
         (void)hipFree(save_vel);
         if (err != hipSuccess) return fail(WC_ERR_HIP, "tuning failed: %s", hipGetErrorString(err));
-        int bd = 0, bu = 0;
-        for (int v = 1; v < nv; ++v) {
-            if (td[v] < td[bd]) bd = v;
-            if (tu[v] < tu[bu]) bu = v;
-        }
-        c->density_variant = bd;
-        c->update_variant = bu;
+        int best[2][2] = {{0, 0}, {0, 0}};
+        for (int mode = 0; mode < 2; ++mode)
+            for (int pass = 0; pass < 2; ++pass)
+                for (int v = 1; v < nv; ++v)
+                    if (t[mode][pass][v] < t[mode][pass][best[mode][pass]]) best[mode][pass] = v;
+        const double step0 = t[0][0][best[0][0]] + t[0][1][best[0][1]];
+        const double step1 = t[1][0][best[1][0]] + t[1][1][best[1][1]];
+        const int mode = step1 < step0 ? 1 : 0;
+        c->density_variant = best[mode][0];
+        c->update_variant = best[mode][1];
+        c->neighbour_list = mode == 1;
         drop_graph(c);
     }
     if (chosen) {
         chosen->density_variant = c->density_variant;
         chosen->update_variant = c->update_variant;
+        chosen->neighbour_list = c->neighbour_list ? 1 : 0;
     }
     for (int v = 0; v < nv; ++v) {
-        if (ms_density) ms_density[v] = td[v];
-        if (ms_update) ms_update[v] = tu[v];
+        if (ms_density) ms_density[v] = t[0][0][v];
+        if (ms_update) ms_update[v] = t[0][1][v];
+        if (ms_density_list) ms_density_list[v] = t[1][0][v];
+        if (ms_update_list) ms_update_list[v] = t[1][1][v];
     }
     return WC_OK;
 }
@@ -1224,7 +1265,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
     auto sort_density = [&]() -> int {
         int r = enqueue_sort(c);
         if (r) return r;
-        WC_HIP(wc::launch_density(c->b, c->density_variant, c->stream));
+        WC_HIP(wc::launch_density(c->b, c->density_variant, c->neighbour_list, c->stream));
         return WC_OK;
     };
     auto judge = [&](float* now) -> int {

@@ -23,6 +23,12 @@ constexpr uint32_t kLookbackSpinLimit = 1u << 24;  // bounded wait in the scan l
 // Records allocated (and zeroed) past the last particle of the particle buffers: the neighbour
 // passes load a cell's slots CHUNK at a time and may read up to CHUNK - 1 records beyond it.
 constexpr size_t kRecordPad = 8;
+// Neighbour lists (optional, chosen by the tuner): the density pass stores each particle's
+// accepted neighbours in enumeration order, entry k of slot t at nlist[k * total + t], up to
+// kNeighbourListMax of them; ncount[t] is their number, or kListOverflow when there were more
+// (the force pass then searches for that particle itself).
+constexpr uint32_t kNeighbourListMax = 64;
+constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
 
 // Per-scene constants, precomputed on the host in float so the device sees exactly the bits the
 // CPU transcription uses.
@@ -90,6 +96,8 @@ struct DeviceBuffers {
     unsigned long long* step_start;   // wall clock when the last step's hash pass started
     uint32_t* rho_max;                // bits of the largest density of the last density pass
     uint32_t* ids[2];                 // particle identity per slot of buffer 1 / 2 (NULL: not tracked)
+    uint32_t* nlist;                  // neighbour lists (NULL until the list mode is first used)
+    uint32_t* ncount;
     Scenes sc;
 };
 
@@ -120,8 +128,9 @@ hipError_t launch_hash(const DeviceBuffers& b, hipStream_t stream);
 hipError_t launch_scan(const DeviceBuffers& b, hipStream_t stream);
 hipError_t launch_place(const DeviceBuffers& b, hipStream_t stream);
 hipError_t launch_gather(const DeviceBuffers& b, hipStream_t stream);
-hipError_t launch_density(const DeviceBuffers& b, int variant, hipStream_t stream);
-hipError_t launch_update(const DeviceBuffers& b, int variant, hipStream_t stream);
+// With `list`, the density pass also writes the neighbour lists and the force pass reads them.
+hipError_t launch_density(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
+hipError_t launch_update(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n,
                                 hipStream_t stream);
 hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,

@@ -431,11 +431,16 @@ __device__ __forceinline__ float eos(const StepConsts& c, float rho) {
     return c.p0 + c.k * (((x * x) * x) - 1.0f);
 }
 
-template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN>
+// LIST: the LDS list holds slots instead; the drain recomputes each squared distance (the same
+// expression, so the same bits) and also appends the slot to the particle's neighbour list in
+// HBM, loading neighbour k + 1's position while neighbour k is summed.
+template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, bool LIST>
 __global__ __launch_bounds__(BLOCK) void k_density(Scenes S, float4* __restrict__ posrho, float4* __restrict__ velp,
                                                    const uint32_t* __restrict__ skey,
-                                                   const uint2* __restrict__ range, uint32_t* __restrict__ rho_max) {
-    __shared__ float s_list[CAP * BLOCK];
+                                                   const uint2* __restrict__ range, uint32_t* __restrict__ rho_max,
+                                                   uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount) {
+    using Entry = typename std::conditional<LIST, uint32_t, float>::type;
+    __shared__ Entry s_list[CAP * BLOCK];
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
@@ -444,13 +449,32 @@ __global__ __launch_bounds__(BLOCK) void k_density(Scenes S, float4* __restrict_
     if (lt >= c.n) return;
     const uint32_t t = d.p_off + lt;
     const float4 p = posrho[t];
-    const lds_ptr<float> list = (lds_ptr<float>)s_list + threadIdx.x;
+    const lds_ptr<Entry> list = (lds_ptr<Entry>)s_list + threadIdx.x;
     float rho = c.m * poly6(c, 0.0f);
+    uint32_t written = 0;  // LIST: neighbours found so far
     auto drain = [&](int cnt) {
-        for (int k = 0; k < cnt; ++k) rho += c.m * poly6(c, sqrtf(list[k * BLOCK]));
+        if constexpr (LIST) {
+            if (cnt <= 0) return;
+            uint32_t u = list[0];
+            float4 q = posrho[u];
+            for (int k = 0; k < cnt; ++k) {
+                const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
+                const float4 qn = posrho[un];
+                const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                const float d2 = (rx * rx + ry * ry) + rz * rz;
+                rho += c.m * poly6(c, sqrtf(d2));
+                if (written < kNeighbourListMax) nlist[(size_t)written * S.total_particles + t] = u;
+                ++written;
+                u = un;
+                q = qn;
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) rho += c.m * poly6(c, sqrtf(list[k * BLOCK]));
+        }
     };
     search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho,
                                                       range + d.c_off, c, list, drain);
+    if constexpr (LIST) ncount[t] = written <= kNeighbourListMax ? written : kListOverflow;
     const float wall = c.wall_density ? wall_density(c, p.x, p.y, p.z) : 0.0f;
     const float density = rho + wall;
     reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = density;
@@ -615,7 +639,9 @@ __device__ __forceinline__ void learned_integrate(const StepConsts& c, const For
 // acc_out the plain acceleration is also written out.  MODE 2: as MODE 1 with the correction
 // averaged over the particle and its accepted neighbours, weights h - distance (h for the
 // particle itself); neighbours within 2h of a wall count with a zero correction.
-template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, int MODE>
+// LIST: the neighbours come from the lists the density pass wrote (a particle whose list
+// overflowed searches as without LIST).
+template <int BLOCK, int CAP, int CHUNK, bool PLANE_DRAIN, int MODE, bool LIST>
 __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __restrict__ posrho_in,
                                                   const float4* __restrict__ velp_in,
                                                   const uint32_t* __restrict__ skey,
@@ -624,7 +650,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                                                   unsigned long long* __restrict__ scene_done,
                                                   const uint32_t* __restrict__ id_sorted, uint32_t* __restrict__ id_out,
                                                   float dt_arg, const float4* __restrict__ corr, float corr_scale,
-                                                  float4* __restrict__ acc_out) {
+                                                  float4* __restrict__ acc_out, const uint32_t* __restrict__ nlist,
+                                                  const uint32_t* __restrict__ ncount) {
     __shared__ uint32_t s_list[CAP * BLOCK];
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
@@ -663,8 +690,30 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
             qv = qvn;
         }
     };
-    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho_in,
-                                                      range + d.c_off, c, list, drain);
+    bool searched = true;
+    if constexpr (LIST) {
+        const uint32_t cnt = ncount[t];
+        if (cnt != kListOverflow) {
+            searched = false;
+            // the same loop as the drain, over the stored list (entry k at nlist[k * total + t])
+            const size_t stride = S.total_particles;
+            if (cnt > 0) {
+                uint32_t u = nlist[t];
+                float4 q = posrho_in[u], qv = velp_in[u];
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const uint32_t un = nlist[(k + 1 < cnt ? k + 1 : k) * stride + t];
+                    const float4 qn = posrho_in[un], qvn = velp_in[un];
+                    f.add(c, p, pv, q, qv);
+                    u = un;
+                    q = qn;
+                    qv = qvn;
+                }
+            }
+        }
+    }
+    if (searched)
+        search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho_in,
+                                                          range + d.c_off, c, list, drain);
 
     V3 ext{c.g[0] * p.w, c.g[1] * p.w, c.g[2] * p.w};
     V3 mouse{0.0f, 0.0f, 0.0f}, wall{0.0f, 0.0f, 0.0f};
@@ -915,31 +964,44 @@ static unsigned pass_blocks(const Scenes& S, int block) {
     return block == 64 ? S.blocks64 : (block == 128 ? S.blocks128 : S.blocks256);
 }
 
-hipError_t launch_density(const DeviceBuffers& b, int v, hipStream_t s) {
+hipError_t launch_density(const DeviceBuffers& b, int v, bool list, hipStream_t s) {
     const PassVariant pv = variant(v);
     const unsigned blocks = pass_blocks(b.sc, pv.block);
     if (blocks == 0) return hipSuccess;
-#define WC_LAUNCH(B, C, K, P)                                                                        \
-    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                \
-        hipLaunchKernelGGL((k_density<B, C, K, P>), dim3(blocks), dim3(B), 0, s, b.sc, b.posrho[1], \
-                           b.velp[1], b.skey, b.range, b.rho_max);                                  \
-        return hipGetLastError();                                                                   \
+    if (list && (!b.nlist || !b.ncount)) return hipErrorInvalidValue;
+#define WC_LAUNCH(B, C, K, P)                                                                             \
+    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                     \
+        if (list)                                                                                         \
+            hipLaunchKernelGGL((k_density<B, C, K, P, true>), dim3(blocks), dim3(B), 0, s, b.sc, b.posrho[1], \
+                               b.velp[1], b.skey, b.range, b.rho_max, b.nlist, b.ncount);                  \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_density<B, C, K, P, false>), dim3(blocks), dim3(B), 0, s, b.sc, b.posrho[1],\
+                               b.velp[1], b.skey, b.range, b.rho_max, nullptr, nullptr);                   \
+        return hipGetLastError();                                                                         \
     }
     WC_VARIANTS(WC_LAUNCH)
 #undef WC_LAUNCH
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_update(const DeviceBuffers& b, int v, hipStream_t s) {
+hipError_t launch_update(const DeviceBuffers& b, int v, bool list, hipStream_t s) {
     const PassVariant pv = variant(v);
     const unsigned blocks = pass_blocks(b.sc, pv.block);
     if (blocks == 0) return hipSuccess;
-#define WC_LAUNCH(B, C, K, P)                                                                        \
-    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                \
-        hipLaunchKernelGGL((k_update<B, C, K, P, 0>), dim3(blocks), dim3(B), 0, s, b.sc, b.posrho[1],\
-                           b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],\
-                           b.ids[0], 0.0f, nullptr, 0.0f, nullptr);                                 \
-        return hipGetLastError();                                                                   \
+    if (list && (!b.nlist || !b.ncount)) return hipErrorInvalidValue;
+#define WC_LAUNCH(B, C, K, P)                                                                             \
+    if (pv.block == B && pv.list_cap == C && pv.chunk == K && pv.plane_drain == P) {                     \
+        if (list)                                                                                         \
+            hipLaunchKernelGGL((k_update<B, C, K, P, 0, true>), dim3(blocks), dim3(B), 0, s, b.sc,         \
+                               b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
+                               b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, b.nlist,    \
+                               b.ncount);                                                                 \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_update<B, C, K, P, 0, false>), dim3(blocks), dim3(B), 0, s, b.sc,        \
+                               b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
+                               b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, nullptr,    \
+                               nullptr);                                                                  \
+        return hipGetLastError();                                                                         \
     }
     WC_VARIANTS(WC_LAUNCH)
 #undef WC_LAUNCH
@@ -958,13 +1020,15 @@ hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* 
                                  float* acc_out, bool smooth, hipStream_t s) {
     if (b.sc.blocks64 == 0) return hipSuccess;
     if (smooth)
-        hipLaunchKernelGGL((k_update<64, 32, 4, true, 2>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc, b.posrho[1],
-                           b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1], b.ids[0], dt,
-                           reinterpret_cast<const float4*>(corr), corr_scale, reinterpret_cast<float4*>(acc_out));
+        hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
+                           b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
+                           b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr);
     else
-        hipLaunchKernelGGL((k_update<64, 32, 4, true, 1>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc, b.posrho[1],
-                           b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1], b.ids[0], dt,
-                           reinterpret_cast<const float4*>(corr), corr_scale, reinterpret_cast<float4*>(acc_out));
+        hipLaunchKernelGGL((k_update<64, 32, 4, true, 1, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
+                           b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
+                           b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr);
     return hipGetLastError();
 }
 

@@ -118,19 +118,25 @@ class Fluid:
         return {L.lib().wc_pass_name(k).decode(): ms[k] for k in range(L.WC_NUM_PASSES)}
 
     def tune(self, dt: float, reps: int = 5) -> dict:
-        """Time every variant of the density and force passes on the current state and keep the
-        fastest (``wc_tune``); the particle state is left as it was."""
+        """Time every variant of the density and force passes on the current state, without and
+        with neighbour lists, and keep the fastest (``wc_tune``); the particle state is left as
+        it was."""
         nv = L.lib().wc_num_variants()
-        md = (C.c_double * nv)()
-        mu = (C.c_double * nv)()
+        md, mu, mdl, mul = ((C.c_double * nv)() for _ in range(4))
         t = L.wc_tuning()
-        L.check(L.lib().wc_tune(self.ctx, C.c_float(dt), reps, C.byref(t), md, mu), "wc_tune")
+        L.check(L.lib().wc_tune(self.ctx, C.c_float(dt), reps, C.byref(t), md, mu, mdl, mul), "wc_tune")
         return {"density_variant": t.density_variant, "update_variant": t.update_variant,
-                "density_ms": list(md), "update_ms": list(mu), "variants": L.variants()}
+                "neighbour_list": t.neighbour_list, "density_ms": list(md), "update_ms": list(mu),
+                "density_list_ms": list(mdl), "update_list_ms": list(mul), "variants": L.variants()}
 
-    def set_tuning(self, density_variant: int, update_variant: int) -> None:
-        t = L.wc_tuning(density_variant, update_variant)
+    def set_tuning(self, density_variant: int, update_variant: int, neighbour_list: int = 0) -> None:
+        t = L.wc_tuning(density_variant, update_variant, neighbour_list)
         L.check(L.lib().wc_set_tuning(self.ctx, C.byref(t)), "wc_set_tuning")
+
+    def get_tuning(self) -> tuple[int, int, int]:
+        t = L.wc_tuning()
+        L.check(L.lib().wc_get_tuning(self.ctx, C.byref(t)), "wc_get_tuning")
+        return t.density_variant, t.update_variant, t.neighbour_list
 
     def track_ids(self, enable: bool = True) -> None:
         L.check(L.lib().wc_track_ids(self.ctx, 1 if enable else 0), "wc_track_ids")

@@ -26,7 +26,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
     ap.add_argument("--tuning", default="auto",
-                    help="'auto' runs the autotuner first; 'D,U' fixes the density / force-pass variants")
+                    help="'auto' runs the autotuner first; 'D,U[,L]' fixes the density / force-pass variants "
+                         "(and L = 1 the neighbour lists)")
     ap.add_argument("--closure-k", type=int, default=0,
                     help="K > 0: after the plain steps, run --steps judged learned steps of K*dt "
                          "(wc_closure_run with the default weights)")
@@ -41,8 +42,7 @@ def main() -> int:
         if args.tuning == "auto":
             f.tune(dt, reps=3)
         else:
-            d, u = (int(x) for x in args.tuning.split(","))
-            f.set_tuning(d, u)
+            f.set_tuning(*(int(x) for x in args.tuning.split(",")))
         f.step(dt, args.warmup)
         f.synchronize()
         t0 = time.perf_counter()
