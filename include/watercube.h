@@ -247,6 +247,11 @@ WC_API int wc_variant_info(int v, int32_t* block, int32_t* list_cap, int32_t* ch
 WC_API int wc_get_tuning(const wc_ctx* ctx, wc_tuning* out);
 WC_API int wc_set_tuning(wc_ctx* ctx, const wc_tuning* tuning);
 
+/* Number of accepted neighbours of every sorted particle (slot order, n entries) from the last
+ * density pass run with neighbour lists; 0xFFFFFFFF marks a particle with more than 64 (its list
+ * was not kept).  WC_ERR_INVALID_ARG when the context has no neighbour lists. */
+WC_API int wc_get_neighbour_counts(wc_ctx* ctx, uint32_t* host, uint32_t n);
+
 /* Autotuner: sorts the current state once, then times every variant of the density pass and of
  * the force pass (reps launches each, HIP events on the context's stream), both without and with
  * neighbour lists, and keeps the fastest variant pair of the faster mode for later steps.  dt > 0

@@ -342,3 +342,41 @@ def test_resume_from_snapshot_equals_uninterrupted_run(tmp_path, oracle):
     q2, rec2, meta2 = snapshot.load(tmp_path / "b")
     assert meta2["step"] == 80 and bytes(q2) == bytes(q)
     _check_records(rec2, st.buf1, "resumed (C++ harness)")
+
+
+def test_neighbour_lists_hold_every_accepted_neighbour(oracle):
+    """With neighbour lists, the count stored for each sorted particle equals the number of other
+    particles with !(d2 >= cut2) -- the density pass's acceptance test, in the same float32
+    arithmetic -- counted by brute force over all pairs (2,048 particles, every one inside the
+    grid, so the 27 cells hold every neighbour)."""
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    import make_golden as G
+
+    d = G.load("dam_break_2k")
+    p = G.params_from_fixture(d["params"])
+    with Fluid(p, d["input"]) as f:
+        f.set_tuning(2, 2, 1)
+        f.sort_.run()
+        f.runDensityProg()
+        got = f.neighbour_counts()
+        pos = f.particles(2)["position"]
+    f32 = np.float32
+    h = f32(p.kernel_radius)
+    cut2 = f32(h * h)
+    while np.sqrt(np.nextafter(cut2, f32(0))) >= h:
+        cut2 = np.nextafter(cut2, f32(0))
+    while np.sqrt(cut2) < h:
+        cut2 = np.nextafter(cut2, f32(np.inf))
+    want = np.empty(len(pos), np.uint32)
+    for i in range(len(pos)):
+        r = pos[i] - pos  # float32
+        d2 = (r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2]
+        ok = ~(d2 >= cut2)
+        ok[i] = False
+        want[i] = ok.sum()
+    want = np.where(want > 64, 0xFFFFFFFF, want).astype(np.uint32)
+    assert np.array_equal(got, want)
+    assert 20 < float(want[want != 0xFFFFFFFF].mean()) < 40

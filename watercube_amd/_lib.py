@@ -146,6 +146,7 @@ SIGNATURES = {
     "wc_pass_name": (C.c_char_p, [C.c_int]),
     "wc_num_variants": (C.c_int, []),
     "wc_variant_info": (C.c_int, [C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),
+    "wc_get_neighbour_counts": (C.c_int, [_vp, _u32p, C.c_uint32]),
     "wc_get_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
     "wc_set_tuning": (C.c_int, [_vp, _P(wc_tuning)]),
     "wc_tune": (C.c_int, [_vp, C.c_float, C.c_uint32, _P(wc_tuning), _P(C.c_double), _P(C.c_double),

@@ -824,6 +824,17 @@ WC_API int wc_set_tuning(wc_ctx* c, const wc_tuning* t) {
     return WC_OK;
 }
 
+WC_API int wc_get_neighbour_counts(wc_ctx* c, uint32_t* host, uint32_t n) {
+    if (!c || (!host && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (!c->b.ncount) return fail(WC_ERR_INVALID_ARG, "the context has no neighbour lists");
+    if (n != c->total_particles) return fail(WC_ERR_INVALID_ARG, "n = %u, the context holds %u", n, c->total_particles);
+    int rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(hipMemcpyAsync(host, c->b.ncount, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
 WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double* ms_density, double* ms_update,
                    double* ms_density_list, double* ms_update_list) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");

@@ -133,6 +133,14 @@ class Fluid:
         t = L.wc_tuning(density_variant, update_variant, neighbour_list)
         L.check(L.lib().wc_set_tuning(self.ctx, C.byref(t)), "wc_set_tuning")
 
+    def neighbour_counts(self) -> np.ndarray:
+        """Accepted neighbours per sorted particle from the last density pass with neighbour
+        lists (0xFFFFFFFF: more than the list holds)."""
+        n = self.params.num_particles
+        out = np.empty(n, np.uint32)
+        L.check(L.lib().wc_get_neighbour_counts(self.ctx, L.u32ptr(out), n), "wc_get_neighbour_counts")
+        return out
+
     def get_tuning(self) -> tuple[int, int, int]:
         t = L.wc_tuning()
         L.check(L.lib().wc_get_tuning(self.ctx, C.byref(t)), "wc_get_tuning")

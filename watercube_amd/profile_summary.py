@@ -34,13 +34,27 @@ def _one(d: Path, pattern: str) -> Path:
     return hits[0]
 
 
-def kernel_stats(trace_dir: Path) -> dict:
+def kernel_stats(trace_dir: Path, last: int = 0) -> dict:
+    """Per-kernel call count and durations: the stats file, or with `last` > 0 the last `last`
+    dispatches of each kernel in the per-dispatch trace."""
     out = {}
-    with open(_one(trace_dir, "*kernel_stats.csv")) as fh:
-        for r in csv.DictReader(fh):
-            out[r["Name"]] = {"calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"]),
-                              "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]),
-                              "percent": float(r["Percentage"])}
+    if last <= 0:
+        with open(_one(trace_dir, "*kernel_stats.csv")) as fh:
+            for r in csv.DictReader(fh):
+                out[r["Name"]] = {"calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"]),
+                                  "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]),
+                                  "percent": float(r["Percentage"])}
+        return out
+    durs = collections.defaultdict(list)
+    with open(_one(trace_dir, "*kernel_trace.csv")) as fh:
+        rows = sorted(csv.DictReader(fh), key=lambda r: int(r["Start_Timestamp"]))
+    for r in rows:
+        durs[r["Kernel_Name"]].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    total = sum(sum(v[-last:]) for v in durs.values())
+    for k, v in durs.items():
+        v = v[-last:]
+        out[k] = {"calls": len(v), "avg_ns": sum(v) / len(v), "min_ns": float(min(v)), "max_ns": float(max(v)),
+                  "percent": 100.0 * sum(v) / max(total, 1)}
     return out
 
 
@@ -54,8 +68,8 @@ def counters(pmc_dir: Path) -> dict:
 
 
 def summarise(trace: Path, fetch: Path | None, write: Path | None, sq: Path | None, workload: str, particles: int,
-              command: str, out: Path, kernels=STEP_KERNELS) -> dict:
-    ks = kernel_stats(trace)
+              command: str, out: Path, kernels=STEP_KERNELS, last: int = 0) -> dict:
+    ks = kernel_stats(trace, last)
     out.mkdir(parents=True, exist_ok=True)
     traffic = {"workload": workload, "particles": particles, "command": command, "kernels": {}}
     if fetch is not None and write is not None:
@@ -137,9 +151,10 @@ def main(argv=None) -> int:
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--command", required=True)
     ap.add_argument("--out", type=Path, default=Path("profiles"))
+    ap.add_argument("--last", type=int, default=0, help="average only each kernel's last N dispatches of the trace")
     a = ap.parse_args(argv)
     t = summarise(a.trace, a.fetch, a.write, a.sq, a.workload, a.particles, a.command, a.out,
-                  kernels=tuple(a.kernels.split(",")))
+                  kernels=tuple(a.kernels.split(",")), last=a.last)
     print(json.dumps({k: round(v.get("hbm_bytes_per_launch", 0) / 1e6, 2) for k, v in t["kernels"].items()}))
     return 0
 
