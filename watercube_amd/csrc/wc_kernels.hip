@@ -695,18 +695,22 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
         const uint32_t cnt = ncount[t];
         if (cnt != kListOverflow) {
             searched = false;
-            // the same loop as the drain, over the stored list (entry k at nlist[k * total + t])
+            // the same loop as the drain, over the stored list (entry k at nlist[k * total + t]):
+            // entry k + 2's index and entry k + 1's records load while entry k's terms are formed,
+            // so no gather waits for the index it needs
             const size_t stride = S.total_particles;
             if (cnt > 0) {
-                uint32_t u = nlist[t];
-                float4 q = posrho_in[u], qv = velp_in[u];
+                const uint32_t last = cnt - 1;
+                const uint32_t u0 = nlist[t];
+                uint32_t u1 = nlist[(size_t)(last < 1u ? last : 1u) * stride + t];
+                float4 q = posrho_in[u0], qv = velp_in[u0];
                 for (uint32_t k = 0; k < cnt; ++k) {
-                    const uint32_t un = nlist[(k + 1 < cnt ? k + 1 : k) * stride + t];
-                    const float4 qn = posrho_in[un], qvn = velp_in[un];
+                    const uint32_t u2 = nlist[(size_t)(k + 2 < cnt ? k + 2 : last) * stride + t];
+                    const float4 qn = posrho_in[u1], qvn = velp_in[u1];
                     f.add(c, p, pv, q, qv);
-                    u = un;
                     q = qn;
                     qv = qvn;
+                    u1 = u2;
                 }
             }
         }
