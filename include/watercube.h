@@ -331,6 +331,12 @@ WC_API int wc_closure_update(wc_ctx* ctx, float dt, const void* corr_dev, float 
  * (enable = 0). */
 WC_API int wc_closure_set_smoothing(wc_ctx* ctx, int enable);
 
+/* How wc_closure_run advances a learned step of K * dt with acceleration a (plain plus
+ * correction): substeps = 0 (the default), one symplectic Euler step, v += K dt a, x += K dt v;
+ * substeps != 0, the outcome of K symplectic Euler steps of dt under a frozen a,
+ * v += K dt a, x += dt (K v_old + K (K+1)/2 dt a). */
+WC_API int wc_closure_set_integrator(wc_ctx* ctx, int substeps);
+
 typedef struct wc_closure_stats {
     uint32_t macro_steps;    /* learned steps of K * dt taken */
     uint32_t fallbacks;      /* of them replaced by K plain steps */
@@ -341,7 +347,8 @@ typedef struct wc_closure_stats {
 /* nmacro learned steps of K * dt.  Each: sort, density, closure inputs, network, force pass with
  * the correction.  Judge: after every learned step the new state's density maximum may exceed
  * the one before the step by at most judge_tol * rho0; otherwise the step is replaced by K plain
- * steps of dt from the saved state. */
+ * steps of dt from the saved state.  net = NULL takes the same judged large steps with no
+ * correction (no closure inputs, no network). */
 WC_API int wc_closure_run(wc_ctx* ctx, wc_mlp* net, float dt, uint32_t K, uint32_t nmacro, float judge_tol,
                           wc_closure_stats* stats);
 

@@ -225,3 +225,41 @@ def test_neighbour_averaged_correction_of_a_constant_field():
     assert deep.sum() > 100 and band.sum() > 100
     assert np.abs(v_avg[deep] - v_own[deep]).max() < 1e-3 * scale
     assert np.abs(v_avg[band] - v_own[band]).max() > 0.05 * scale
+
+
+def test_frozen_force_integrator_matches_its_formula():
+    """wc_closure_run with no network and the substep integrator advances each particle as K
+    symplectic Euler steps of dt under the force frozen at the start: v = v0 + (K dt) a,
+    x = x0 + dt (K v0 + (K (K+1)/2) dt a), in float32 in that association (checked bit for bit
+    where no wall clamp applies; a from wc_closure_update's plain acceleration)."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    K = 4
+    n = p.num_particles
+    f32 = np.float32
+    with Fluid(p, parts) as a:
+        a.step(dt, 20)
+        state = a.particles(1)
+    acc = DeviceArray((n, 4), np.float32)
+    with Fluid(p, state) as f:
+        f.sort_.run()
+        f.runDensityProg()
+        s0 = f.particles(2)
+        L.check(L.lib().wc_closure_update(f.ctx, C.c_float(dt), None, C.c_float(0.0), acc.ptr), "closure update")
+        f.synchronize()
+        a_plain = acc.download()[:, :3]
+    with Fluid(p, state) as g:
+        L.check(L.lib().wc_closure_set_integrator(g.ctx, 1), "integrator")
+        st = L.wc_closure_stats()
+        L.check(L.lib().wc_closure_run(g.ctx, None, C.c_float(dt), K, 1, C.c_float(1e30), C.byref(st)), "run")
+        out = g.particles(1)
+    dt_large = f32(dt) * f32(K)
+    kf = f32(K)
+    dts = dt_large / kf
+    s = (kf * (kf + f32(1.0))) * f32(0.5)
+    x0, v0 = s0["position"], s0["velocity"]
+    want_x = x0 + dts * (kf * v0 + (s * dts) * a_plain)
+    want_v = v0 + a_plain * dt_large
+    free = np.all((want_x > f32(p.border)) & (want_x < f32(p.size) - f32(p.border)), axis=1)
+    assert free.sum() > 0.9 * n
+    assert np.array_equal(out["position"][free].view(np.uint32), want_x[free].view(np.uint32))
+    assert np.array_equal(out["velocity"][free].view(np.uint32), want_v[free].view(np.uint32))

@@ -171,6 +171,7 @@ SIGNATURES = {
     "wc_closure_features": (C.c_int, [_vp, C.c_float, _u32p, C.c_uint32, _vp]),
     "wc_closure_update": (C.c_int, [_vp, C.c_float, _vp, C.c_float, _vp]),
     "wc_closure_set_smoothing": (C.c_int, [_vp, C.c_int]),
+    "wc_closure_set_integrator": (C.c_int, [_vp, C.c_int]),
     "wc_closure_run": (C.c_int, [_vp, _vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, _P(wc_closure_stats)]),
     "wc_last_error": (C.c_char_p, []),
     "wc_abi_version": (C.c_int, []),

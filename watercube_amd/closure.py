@@ -308,11 +308,30 @@ def _state_by_id(f: Fluid) -> np.ndarray:
     return out
 
 
+def _closure_run(f: Fluid, net: "ClosureNet | None", dt: float, K: int, macro: int, judge_tol: float,
+                 smooth: bool, substeps: bool) -> L.wc_closure_stats:
+    L.check(L.lib().wc_closure_set_smoothing(f.ctx, int(smooth)), "wc_closure_set_smoothing")
+    L.check(L.lib().wc_closure_set_integrator(f.ctx, int(substeps)), "wc_closure_set_integrator")
+    st = L.wc_closure_stats()
+    L.check(L.lib().wc_closure_run(f.ctx, net.h if net is not None else None, C.c_float(dt), K, macro,
+                                   C.c_float(judge_tol), C.byref(st)), "wc_closure_run")
+    return st
+
+
+# Large-step methods compared against K plain steps of dt: (network?, substep integrator?)
+METHODS = {
+    "plain_large": (False, False),    # one symplectic Euler step of K dt (graph step)
+    "frozen_force": (False, True),    # K steps of dt under the force frozen at the step's start
+    "learned": (True, False),         # one step of K dt with the learned correction
+    "learned_frozen": (True, True),   # K frozen-force steps of dt with the learned correction
+}
+
+
 def one_step_errors(net: ClosureNet, p, parts: np.ndarray, dt: float, K: int, starts=(0, 30, 60, 90),
                     variants=None, smooth: bool = True) -> dict:
-    """From states along a plain small-dt run: one plain step of K*dt and one learned step of K*dt
-    (judge disabled) against K plain steps of dt.  RMS velocity and position errors over the
-    particles at least 2h from every wall, per start and averaged."""
+    """From states along a plain small-dt run: one step of K*dt by each method (judge disabled)
+    against K plain steps of dt.  RMS velocity and position errors over the particles at least
+    2h from every wall, per start and averaged, and their ratios to the plain large step's."""
     h = float(p.kernel_radius)
     size = float(p.size)
 
@@ -336,35 +355,34 @@ def one_step_errors(net: ClosureNet, p, parts: np.ndarray, dt: float, K: int, st
             with fluid(state) as a:
                 a.step(dt, K)
                 ref = _state_by_id(a)
-            with fluid(state) as b:
-                b.step(dt * K, 1)
-                big = _state_by_id(b)
-            with fluid(state) as c:
-                L.check(L.lib().wc_closure_set_smoothing(c.ctx, int(smooth)), "wc_closure_set_smoothing")
-                st = L.wc_closure_stats()
-                L.check(L.lib().wc_closure_run(c.ctx, net.h, C.c_float(dt), K, 1, C.c_float(1e30), C.byref(st)),
-                        "wc_closure_run")
-                lrn = _state_by_id(c)
+            out = {}
+            for m, (use_net, sub) in METHODS.items():
+                with fluid(state) as b:
+                    if m == "plain_large":
+                        b.step(dt * K, 1)
+                    else:
+                        _closure_run(b, net if use_net else None, dt, K, 1, 1e30, smooth, sub)
+                    out[m] = _state_by_id(b)
 
             def rms(x, field):
                 d = (x[field] - ref[field])[interior]
                 return float(np.sqrt(np.mean(np.sum(d.astype(np.float64) ** 2, axis=1))))
 
             rows.append({"start": s0, "interior": int(interior.sum()),
-                         "plain_large": {"vel_rms": rms(big, "velocity"), "pos_rms": rms(big, "position")},
-                         "learned": {"vel_rms": rms(lrn, "velocity"), "pos_rms": rms(lrn, "position")}})
-    mean = {k: {m: float(np.mean([r[k][m] for r in rows])) for m in ("vel_rms", "pos_rms")}
-            for k in ("plain_large", "learned")}
-    return {"K": K, "per_start": rows, "mean": mean,
-            "vel_err_ratio": mean["learned"]["vel_rms"] / max(mean["plain_large"]["vel_rms"], 1e-30),
-            "pos_err_ratio": mean["learned"]["pos_rms"] / max(mean["plain_large"]["pos_rms"], 1e-30)}
+                         **{m: {"vel_rms": rms(x, "velocity"), "pos_rms": rms(x, "position")} for m, x in out.items()}})
+    mean = {k: {m: float(np.mean([r[k][m] for r in rows])) for m in ("vel_rms", "pos_rms")} for k in METHODS}
+    base = mean["plain_large"]
+    ratios = {k: {"vel": mean[k]["vel_rms"] / max(base["vel_rms"], 1e-30),
+                  "pos": mean[k]["pos_rms"] / max(base["pos_rms"], 1e-30)} for k in METHODS}
+    return {"K": K, "per_start": rows, "mean": mean, "ratio_to_plain_large": ratios,
+            "vel_err_ratio": ratios["learned"]["vel"], "pos_err_ratio": ratios["learned"]["pos"]}
 
 
 def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 100, ks=(2, 4),
              judge_tol: float = 0.01, smooth: bool = True, log=print) -> dict:
-    """Plain small-dt run vs learned large steps vs plain large steps over `horizon` small steps
-    from the same state: per-particle density error against the small-dt run (particles matched
-    by identity) and simulated time per wall-clock second.  Every run uses the same tuned
+    """Plain small-dt run vs the large-step methods of METHODS over `horizon` small steps from the
+    same state: per-particle density error against the small-dt run (particles matched by
+    identity) and simulated time per wall-clock second.  Every run uses the same tuned
     neighbour-pass variants; graph capture and tuning stay outside the timed regions."""
     p, parts, dt, _ = S.workload(workload)
     n = int(p.num_particles)
@@ -391,8 +409,15 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
         ref = _densities_by_id(f)
     res["plain"] = {"wall_s": t_plain, "sim_s_per_s": horizon * dt / t_plain, "particle_steps_per_s": n * horizon / t_plain}
     log(f"plain dt={dt}: {json.dumps(res['plain'])}")
+
+    def errors(rho):
+        err = np.abs(rho - ref) / rho0
+        return {"mean_density_err": float(err.mean()), "p99_density_err": float(np.percentile(err, 99)),
+                "max_density_err": float(err.max())}
+
     for K in ks:
         macro = horizon // K
+        entry = {}
         with fluid() as f:
             f.step(dt * K, 1)  # captures the step graph
             f.synchronize()
@@ -400,26 +425,20 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
             f.step(dt * K, macro - 1)
             f.synchronize()
             t_large = (time.perf_counter() - t0) * macro / max(1, macro - 1)
-            rho = _densities_by_id(f)
-        err = np.abs(rho - ref) / rho0
-        entry = {"plain_large": {"mean_density_err": float(err.mean()), "p99_density_err": float(np.percentile(err, 99)),
-                                 "max_density_err": float(err.max()), "wall_s": t_large,
-                                 "sim_s_per_s": macro * K * dt / t_large,
-                                 "speedup_vs_plain": (macro * K * dt / t_large) / res["plain"]["sim_s_per_s"]}}
-        if net is not None:
+            entry["plain_large"] = {**errors(_densities_by_id(f)), "wall_s": t_large,
+                                    "sim_s_per_s": macro * K * dt / t_large,
+                                    "speedup_vs_plain": (macro * K * dt / t_large) / res["plain"]["sim_s_per_s"]}
+        for m, (use_net, sub) in METHODS.items():
+            if m == "plain_large" or (use_net and net is None):
+                continue
             with fluid() as f:
-                L.check(L.lib().wc_closure_set_smoothing(f.ctx, int(smooth)), "wc_closure_set_smoothing")
-                st = L.wc_closure_stats()
-                L.check(L.lib().wc_closure_run(f.ctx, net.h, C.c_float(dt), K, macro, C.c_float(judge_tol),
-                                               C.byref(st)), "wc_closure_run")
+                st = _closure_run(f, net if use_net else None, dt, K, macro, judge_tol, smooth, sub)
                 rho = _densities_by_id(f)
-            err = np.abs(rho - ref) / rho0
             wall = st.wall_ms * 1e-3
-            entry["learned"] = {"mean_density_err": float(err.mean()), "p99_density_err": float(np.percentile(err, 99)),
-                                "max_density_err": float(err.max()), "macro_steps": int(st.macro_steps),
-                                "fallbacks": int(st.fallbacks), "wall_s": wall, "sim_s_per_s": macro * K * dt / wall,
-                                "speedup_vs_plain": (macro * K * dt / wall) / res["plain"]["sim_s_per_s"],
-                                "max_density_jump": float(st.max_density_jump)}
+            entry[m] = {**errors(rho), "macro_steps": int(st.macro_steps), "fallbacks": int(st.fallbacks),
+                        "wall_s": wall, "sim_s_per_s": macro * K * dt / wall,
+                        "speedup_vs_plain": (macro * K * dt / wall) / res["plain"]["sim_s_per_s"],
+                        "max_density_jump": float(st.max_density_jump)}
         if net is not None:
             entry["one_step"] = one_step_errors(net, p, parts, dt, K, variants=variants, smooth=smooth)
         res[f"K{K}"] = entry

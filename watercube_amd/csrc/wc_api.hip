@@ -128,6 +128,7 @@ struct wc_ctx {
     uint32_t* cl_rows = nullptr;    // row list for feature requests
     uint32_t cl_rows_cap = 0;
     bool cl_smooth = true;          // average the correction over neighbours in wc_closure_run
+    bool cl_substeps = false;       // integrate a learned step as K frozen-force substeps
 };
 
 struct wc_mlp {
@@ -1229,7 +1230,7 @@ WC_API int wc_closure_update(wc_ctx* c, float dt, const void* corr_dev, float co
     if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "the learned closure runs on single-scene contexts");
     int rc = set_device(c);
     if (rc) return rc;
-    WC_HIP(wc::launch_closure_update(c->b, dt, (const float*)corr_dev, corr_scale, (float*)acc_dev, false,
+    WC_HIP(wc::launch_closure_update(c->b, dt, (const float*)corr_dev, corr_scale, (float*)acc_dev, false, 0.0f,
                                      c->stream));
     return WC_OK;
 }
@@ -1248,9 +1249,15 @@ WC_API int wc_closure_set_smoothing(wc_ctx* c, int enable) {
     return WC_OK;
 }
 
+WC_API int wc_closure_set_integrator(wc_ctx* c, int substeps) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "NULL context");
+    c->cl_substeps = substeps != 0;
+    return WC_OK;
+}
+
 WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t nmacro, float judge_tol,
                           wc_closure_stats* stats) {
-    if (!c || !net || !stats) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (!c || !stats) return fail(WC_ERR_INVALID_ARG, "NULL argument");
     if (K == 0) return fail(WC_ERR_INVALID_ARG, "K must be >= 1");
     int rc = closure_workspace(c);
     if (rc) return rc;
@@ -1321,11 +1328,15 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
                                c->stream);
         if (e == hipSuccess && c->b.ids[0])
             e = hipMemcpyAsync(c->cl_backup_ids, c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess) e = wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(net->m.stream);
-        if (e == hipSuccess) e = wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream);
+        if (net) {
+            if (e == hipSuccess)
+                e = wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(net->m.stream);
+            if (e == hipSuccess) e = wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream);
+        }
         if (e == hipSuccess)
-            e = wc::launch_closure_update(c->b, dt_large, c->cl_y, g, nullptr, c->cl_smooth, c->stream);
+            e = wc::launch_closure_update(c->b, dt_large, net ? c->cl_y : nullptr, g, nullptr, c->cl_smooth,
+                                          c->cl_substeps ? (float)K : 0.0f, c->stream);
         if (e != hipSuccess) rc = fail(WC_ERR_HIP, "learned step: %s", hipGetErrorString(e));
         stats->macro_steps += 1;
     }

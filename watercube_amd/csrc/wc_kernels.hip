@@ -595,15 +595,25 @@ struct ForceSums {
 // Integration of the learned large step: the reference's update with the network's correction
 // (units of g, times corr_scale) added to the acceleration of particles at least 2h from every
 // wall, and the plain acceleration optionally written out (w = 1 for those particles).
+// substeps = 0: one symplectic Euler step of dt (the reference's update with the correction).
+// substeps = K > 0: the outcome of K symplectic Euler steps of dt / K under a force frozen at its
+// current value, v = v0 + K (dt/K) a and x = x0 + (dt/K) (K v0 + K (K+1)/2 (dt/K) a), which
+// removes the K (K-1)/2 (dt/K)^2 a position error of a single large step.
 __device__ __forceinline__ void learned_axis(const StepConsts& c, float pf, float vf, float ext, float denom,
-                                             float v0, float x0, float dt, float corr, float& acc_plain, float& vel,
-                                             float& pos) {
+                                             float v0, float x0, float dt, float substeps, float corr,
+                                             float& acc_plain, float& vel, float& pos) {
     vf = vf * c.mu;
     const float force = (pf + vf) + ext;
     acc_plain = force / denom;
-    const float v = v0 + (acc_plain + corr) * dt;
+    const float a = acc_plain + corr;
+    const float v = v0 + a * dt;
     vel = fminf(fmaxf(v, -c.max_speed), c.max_speed);
-    pos = x0 + vel * dt;
+    if (substeps > 0.0f) {
+        const float dts = dt / substeps;
+        pos = x0 + dts * (substeps * v0 + (substeps * (substeps + 1.0f) * 0.5f) * dts * a);
+    } else {
+        pos = x0 + vel * dt;
+    }
     if (pos < c.border) {
         vel = vel * -c.damping;
         pos = c.border;
@@ -621,15 +631,15 @@ __device__ __forceinline__ bool closure_interior(const StepConsts& c, float x, f
 
 __device__ __forceinline__ void learned_integrate(const StepConsts& c, const ForceSums& f, const V3& ext, float denom,
                                                   const float4& p, const float4& pv, uint32_t t, float dt,
-                                                  bool have_corr, const V3& kc, float corr_scale,
+                                                  float substeps, bool have_corr, const V3& kc, float corr_scale,
                                                   float4* __restrict__ acc_out, float4& po, float4& vo) {
     float4 k = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool interior = closure_interior(c, p.x, p.y, p.z);
     if (have_corr && interior) k = make_float4(kc.x * corr_scale, kc.y * corr_scale, kc.z * corr_scale, 0.0f);
     float4 a;
-    learned_axis(c, f.pfx, f.vfx, ext.x, denom, pv.x, p.x, dt, k.x, a.x, vo.x, po.x);
-    learned_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, k.y, a.y, vo.y, po.y);
-    learned_axis(c, f.pfz, f.vfz, ext.z, denom, pv.z, p.z, dt, k.z, a.z, vo.z, po.z);
+    learned_axis(c, f.pfx, f.vfx, ext.x, denom, pv.x, p.x, dt, substeps, k.x, a.x, vo.x, po.x);
+    learned_axis(c, f.pfy, f.vfy, ext.y, denom, pv.y, p.y, dt, substeps, k.y, a.y, vo.y, po.y);
+    learned_axis(c, f.pfz, f.vfz, ext.z, denom, pv.z, p.z, dt, substeps, k.z, a.z, vo.z, po.z);
     a.w = interior ? 1.0f : 0.0f;
     if (acc_out) acc_out[t] = a;
 }
@@ -651,7 +661,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                                                   const uint32_t* __restrict__ id_sorted, uint32_t* __restrict__ id_out,
                                                   float dt_arg, const float4* __restrict__ corr, float corr_scale,
                                                   float4* __restrict__ acc_out, const uint32_t* __restrict__ nlist,
-                                                  const uint32_t* __restrict__ ncount) {
+                                                  const uint32_t* __restrict__ ncount, float substeps) {
     __shared__ uint32_t s_list[CAP * BLOCK];
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
@@ -745,7 +755,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                 kc = V3{ct.x, ct.y, ct.z};
             }
         }
-        learned_integrate(c, f, ext, denom, p, pv, t, dt_arg, corr != nullptr, kc, corr_scale, acc_out, po, vo);
+        learned_integrate(c, f, ext, denom, p, pv, t, dt_arg, substeps, corr != nullptr, kc, corr_scale, acc_out, po,
+                          vo);
     }
     po.w = p.w;
     vo.w = pv.w;
@@ -999,12 +1010,12 @@ hipError_t launch_update(const DeviceBuffers& b, int v, bool list, hipStream_t s
             hipLaunchKernelGGL((k_update<B, C, K, P, 0, true>), dim3(blocks), dim3(B), 0, s, b.sc,         \
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
                                b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, b.nlist,    \
-                               b.ncount);                                                                 \
+                               b.ncount, 0.0f);                                                                 \
         else                                                                                              \
             hipLaunchKernelGGL((k_update<B, C, K, P, 0, false>), dim3(blocks), dim3(B), 0, s, b.sc,        \
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
                                b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, nullptr,    \
-                               nullptr);                                                                  \
+                               nullptr, 0.0f);                                                                  \
         return hipGetLastError();                                                                         \
     }
     WC_VARIANTS(WC_LAUNCH)
@@ -1021,18 +1032,18 @@ hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows,
 }
 
 hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
-                                 float* acc_out, bool smooth, hipStream_t s) {
+                                 float* acc_out, bool smooth, float substeps, hipStream_t s) {
     if (b.sc.blocks64 == 0) return hipSuccess;
     if (smooth)
         hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr);
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps);
     else
         hipLaunchKernelGGL((k_update<64, 32, 4, true, 1, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr);
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps);
     return hipGetLastError();
 }
 
