@@ -380,3 +380,27 @@ def test_neighbour_lists_hold_every_accepted_neighbour(oracle):
     want = np.where(want > 64, 0xFFFFFFFF, want).astype(np.uint32)
     assert np.array_equal(got, want)
     assert 20 < float(want[want != 0xFFFFFFFF].mean()) < 40
+
+
+def test_graph_replays_interleaved_with_eager_work(oracle):
+    """The captured step is replayed many times with eager passes and host copies between the
+    replays; every replay's result is checked against the CPU restatement (no replay may see
+    stale scan state, counters or clocks from the one before)."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    st = oracle.OracleState(p, parts)
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, 1)
+        for r in range(6):
+            f.step(dt, 1)          # graph replay
+            st.step(dt, 1)
+            _check_records(f.particles(1), st.buf1, f"replay {r}")
+            f.sort_.run()          # eager work on the same stream between replays
+            f.runDensityProg()
+            f.synchronize()
+            # the eager sort of buffer 1 is the sort the next step will do
+            nxt = oracle.OracleState(p, st.buf1)
+            nxt.sort_run()
+            assert np.array_equal(f.sort_.getOffsetBuffer(), nxt.offsets), f"eager sort after replay {r}"
+        f.step(dt, 3)
+        st.step(dt, 3)
+        _check_records(f.particles(1), st.buf1, "three more replays")
