@@ -188,12 +188,16 @@ __global__ void k_cast(const float* __restrict__ w, uint16_t* __restrict__ rowma
 }
 
 // ------------------------------------------------------------------------------------------
-// Fused inference: Y[n x 4] = MLP(X[n x 208]).  Persistent: one workgroup of four waves per CU
+// Fused inference: Y[n x 4] = MLP(X[n x 208]).  Persistent: one workgroup of eight waves per CU
 // copies the three hidden layers' weights into LDS once (bf16, transposed [out][in], each row
-// padded by 8 values so that the 32 rows an MFMA operand read touches fall in different banks)
-// and then walks tiles of 128 rows; each wave takes 32 rows of a tile through the four layers,
-// its activations in an LDS buffer of its own.  The output layer's weights (8 KB) are read
-// from global memory.
+// padded by 8 values so that the 16 rows an MFMA operand read touches fall in different banks)
+// and then walks tiles of 128 rows; each wave takes 16 rows of a tile through the four layers on
+// v_mfma_f32_16x16x32_bf16, its activations in an LDS buffer of its own.  Two waves per SIMD, so
+// one wave's SiLU arithmetic overlaps the other's matrix products.  The output layer's weights
+// (8 KB) are read from global memory.
+//
+// MFMA 16x16x32 bf16 lane layout (wave64, lane l): A operand = A[l % 16][8 (l / 16) + 0..7],
+// B operand = B[8 (l / 16) + 0..7][l % 16], accumulator element r = C[4 (l / 16) + r][l % 16].
 // ------------------------------------------------------------------------------------------
 struct InferWeights {
     const uint16_t* w1t;  // [128][208]
@@ -203,12 +207,21 @@ struct InferWeights {
     const float *b1, *b2, *b3, *b4;
 };
 
-constexpr int kLd1 = kMlpInPad + 8;   // LDS row length of layer 1's weights
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x4 mfma16(uint4 a, uint4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+constexpr int kLd1 = kMlpInPad + 8;   // LDS row length of layer 1's weights (208 used, 8 zero)
 constexpr int kLdH = kMlpHidden + 8;  // LDS row length of layers 2-3's weights and of activations
-constexpr int kInferWaves = 4;
-constexpr int kInferTile = kInferWaves * 32;  // rows per tile
-constexpr int kInferLdsHalves = kMlpHidden * kLd1 + 2 * kMlpHidden * kLdH + kInferWaves * 32 * kLdH;
+constexpr int kInferWaves = 8;
+constexpr int kInferRows = 16;                         // rows per wave and tile
+constexpr int kInferTile = kInferWaves * kInferRows;   // rows per tile
+constexpr int kKSteps1 = (kMlpInPad + 31) / 32;        // layer 1's k steps of 32 (the last half used)
+constexpr int kInferLdsHalves = kMlpHidden * kLd1 + 2 * kMlpHidden * kLdH + kInferWaves * kInferRows * kLdH;
 static_assert(kInferLdsHalves * 2 <= 160 * 1024, "weights and activations must fit one CU's LDS");
+static_assert(32 * kKSteps1 <= kLd1 + 8, "layer 1's last k step reads at most 8 values past a row");
 
 // LDS written by this wave is visible to this wave's later reads: wait for its LDS operations
 // and keep the compiler from moving memory accesses across.  (Each wave of the inference kernel
@@ -219,54 +232,50 @@ __device__ __forceinline__ void wave_lds_sync() {
     asm volatile("" ::: "memory");
 }
 
-constexpr int kKSteps1 = kMlpInPad / 16;  // k steps of layer 1
-
-// Layer 1's A operand for a wave's 32 rows starting at row0 (rows at or past n read as zeros).
+// Layer 1's A operand for a wave's 16 rows from row0 (rows at or past n, and columns at or past
+// 208, read as zeros).
 __device__ __forceinline__ void load_rows(uint4 (&a)[kKSteps1], const uint16_t* __restrict__ X, long row0, uint32_t n,
                                           int lane) {
-    const int row = lane & 31, kq = (lane >> 5) * 8;
+    const int row = lane & 15, kq = (lane >> 4) * 8;
     const bool live = row0 + row < (long)n;
 #pragma unroll
     for (int s = 0; s < kKSteps1; ++s) a[s] = make_uint4(0u, 0u, 0u, 0u);
     if (live) {
         const uint16_t* src = X + (row0 + row) * kMlpInPad + kq;
 #pragma unroll
-        for (int s = 0; s < kKSteps1; ++s) a[s] = *reinterpret_cast<const uint4*>(src + 16 * s);
+        for (int s = 0; s < kKSteps1; ++s)
+            if (32 * s + kq < kMlpInPad) a[s] = *reinterpret_cast<const uint4*>(src + 32 * s);
     }
 }
 
-// act_out = bf16(silu(A W + b)) for a wave's 32 rows; W^T rows from LDS with stride LDW; A
+// act_out = bf16(silu(A W + b)) for a wave's 16 rows; W^T rows from LDS with stride LDW; A
 // operands from `areg` (KSTEPS registers, layer 1) or from the wave's activation buffer.
 template <int KSTEPS, int LDW, bool FROM_REGS>
 __device__ __forceinline__ void infer_layer(const uint4* areg, const uint16_t* act_in, const uint16_t* __restrict__ wt,
                                             const float* __restrict__ bias, uint16_t* act_out, int lane) {
-    const int row = lane & 31, kq = (lane >> 5) * 8;
-    // accumulators start at the bias: every element of tile j lies in column j * 32 + row
-    f32x16 acc[4];
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    // accumulators start at the bias: every element of tile j lies in column j * 16 + r16
+    f32x4 acc[8];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const float bj = bias[j * 32 + row];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[j][r] = bj;
+    for (int j = 0; j < 8; ++j) {
+        const float bj = bias[j * 16 + r16];
+        acc[j] = f32x4{bj, bj, bj, bj};
     }
 #pragma unroll
     for (int s = 0; s < KSTEPS; ++s) {
-        const uint4 av = FROM_REGS ? areg[s] : *reinterpret_cast<const uint4*>(&act_in[row * kLdH + 16 * s + kq]);
+        const uint4 av = FROM_REGS ? areg[s] : *reinterpret_cast<const uint4*>(&act_in[r16 * kLdH + 32 * s + kq]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint4 bv = *reinterpret_cast<const uint4*>(&wt[(j * 32 + row) * LDW + 16 * s + kq]);
-            acc[j] = mfma(av, bv, acc[j]);
+        for (int j = 0; j < 8; ++j) {
+            const uint4 bv = *reinterpret_cast<const uint4*>(&wt[(j * 16 + r16) * LDW + 32 * s + kq]);
+            acc[j] = mfma16(av, bv, acc[j]);
         }
     }
     // every write below depends on every read above, so act_out may be act_in
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = j * 32 + row;
+    for (int j = 0; j < 8; ++j) {
+        const int col = j * 16 + r16;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            act_out[rr * kLdH + col] = f2bf(silu(acc[j][r]));
-        }
+        for (int r = 0; r < 4; ++r) act_out[(4 * (lane >> 4) + r) * kLdH + col] = f2bf(silu(acc[j][r]));
     }
 }
 
@@ -277,16 +286,19 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(const uint16_t* 
     uint16_t* w2 = w1 + kMlpHidden * kLd1;         // [128][kLdH]
     uint16_t* w3 = w2 + kMlpHidden * kLdH;         // [128][kLdH]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint16_t* act = w3 + kMlpHidden * kLdH + wave * 32 * kLdH;  // this wave's [32][kLdH]
+    uint16_t* act = w3 + kMlpHidden * kLdH + wave * kInferRows * kLdH;  // this wave's [16][kLdH]
 
     const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
     uint4 xa[kKSteps1];
-    if (blockIdx.x < tiles) load_rows(xa, X, (long)blockIdx.x * kInferTile + wave * 32, n, lane);
+    if (blockIdx.x < tiles) load_rows(xa, X, (long)blockIdx.x * kInferTile + wave * kInferRows, n, lane);
 
-    // weights into LDS, 8 values per copy
-    for (int i = tid; i < kMlpHidden * (kMlpInPad / 8); i += kInferWaves * 64) {
-        const int r = i / (kMlpInPad / 8), q = i % (kMlpInPad / 8);
-        *reinterpret_cast<uint4*>(&w1[r * kLd1 + q * 8]) = reinterpret_cast<const uint4*>(W.w1t)[i];
+    // weights into LDS, 8 values per copy; layer 1's row padding is zeroed (its last k step
+    // multiplies it, and the first 8 values of the next row, by zero inputs)
+    for (int i = tid; i < kMlpHidden * (kLd1 / 8); i += kInferWaves * 64) {
+        const int r = i / (kLd1 / 8), q = i % (kLd1 / 8);
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (q < kMlpInPad / 8) v = reinterpret_cast<const uint4*>(W.w1t)[r * (kMlpInPad / 8) + q];
+        *reinterpret_cast<uint4*>(&w1[r * kLd1 + q * 8]) = v;
     }
     for (int i = tid; i < kMlpHidden * (kMlpHidden / 8); i += kInferWaves * 64) {
         const int r = i / (kMlpHidden / 8), q = i % (kMlpHidden / 8);
@@ -298,31 +310,31 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(const uint16_t* 
     // From here on the waves are independent: each walks the workgroup's tiles with its own
     // activation buffer; the next tile's input rows load while the current tile is computed.
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const long row0 = (long)tile * kInferTile + wave * 32;
-        const int rows_left = (int)max(0L, min(32L, (long)n - row0));
+        const long row0 = (long)tile * kInferTile + wave * kInferRows;
+        const int rows_left = (int)max(0L, min((long)kInferRows, (long)n - row0));
         infer_layer<kKSteps1, kLd1, true>(xa, nullptr, w1, W.b1, act, lane);
         const uint32_t next = tile + gridDim.x;
-        if (next < tiles) load_rows(xa, X, (long)next * kInferTile + wave * 32, n, lane);
+        if (next < tiles) load_rows(xa, X, (long)next * kInferTile + wave * kInferRows, n, lane);
         wave_lds_sync();
-        infer_layer<kMlpHidden / 16, kLdH, false>(nullptr, act, w2, W.b2, act, lane);
+        infer_layer<kMlpHidden / 32, kLdH, false>(nullptr, act, w2, W.b2, act, lane);
         wave_lds_sync();
-        infer_layer<kMlpHidden / 16, kLdH, false>(nullptr, act, w3, W.b3, act, lane);
+        infer_layer<kMlpHidden / 32, kLdH, false>(nullptr, act, w3, W.b3, act, lane);
         wave_lds_sync();
-        // output layer: one 32x32 tile, columns 0..2 used
-        f32x16 acc = {};
-        const int row = lane & 31, kq = (lane >> 5) * 8;
+        // output layer: one 16x16 tile, columns 0..2 used
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int r16 = lane & 15, kq = (lane >> 4) * 8;
 #pragma unroll
-        for (int k0 = 0; k0 < kMlpHidden; k0 += 16) {
-            const uint4 av = *reinterpret_cast<const uint4*>(&act[row * kLdH + k0 + kq]);
-            const uint4 bv = *reinterpret_cast<const uint4*>(&W.w4t[(long)row * kMlpHidden + k0 + kq]);
-            acc = mfma(av, bv, acc);
+        for (int s = 0; s < kMlpHidden / 32; ++s) {
+            const uint4 av = *reinterpret_cast<const uint4*>(&act[r16 * kLdH + 32 * s + kq]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&W.w4t[(long)r16 * kMlpHidden + 32 * s + kq]);
+            acc = mfma16(av, bv, acc);
         }
-        const int col = row;  // output column of this lane
+        const int col = r16;  // output column of this lane
         if (col < 4) {
             const float bc = col < kMlpOut ? W.b4[col] : 0.0f;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            for (int r = 0; r < 4; ++r) {
+                const int rr = 4 * (lane >> 4) + r;
                 if (rr < rows_left) Y[(row0 + rr) * 4 + col] = col < kMlpOut ? acc[r] + bc : 0.0f;
             }
         }
