@@ -317,7 +317,9 @@ WC_API int wc_get_ids(wc_ctx* ctx, int which, uint32_t* host, uint32_t n);
 WC_API int wc_max_density(wc_ctx* ctx, float* out);
 
 /* Closure inputs of the sorted particles of buffer 2 (after wc_sort_run and wc_density): rows
- * lists the slots to describe (NULL: all n, in slot order); x_dev receives nrows (or n) rows. */
+ * lists the slots to describe (NULL: all n, in slot order); x_dev receives nrows (or n) rows.
+ * With neighbour lists (wc_tuning.neighbour_list) the neighbours are read from the lists of the
+ * last density pass, which must follow the last sort. */
 WC_API int wc_closure_features(wc_ctx* ctx, float dt_mult, const uint32_t* rows, uint32_t nrows, void* x_dev);
 /* Force pass buffer 2 -> buffer 1 with step dt; corr_dev (n rows of 4 floats, times corr_scale)
  * is added to the acceleration of particles at least 2h from every wall (NULL: none); acc_dev

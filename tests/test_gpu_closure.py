@@ -94,8 +94,9 @@ def _enumeration(oracle, p, pos, counts, offsets, t):
     return out
 
 
+@pytest.mark.parametrize("lists", [0, 1])
 @pytest.mark.parametrize("scene", ["dam_break_16k", "dense_h3"])
-def test_features_match_numpy_reference(scene, oracle):
+def test_features_match_numpy_reference(scene, lists, oracle):
     """dam_break_16k has about 33 neighbours inside h per particle (the 32-of-n selection at the
     end of the walk); dense_h3 (h = 3 spacings) about 113, so the kernel's list also overflows
     and is cut back to the 32 nearest during the walk."""
@@ -105,8 +106,9 @@ def test_features_match_numpy_reference(scene, oracle):
         p, parts = scenes.scaled_block((24, 24, 24), 0.01, seed=5, h_over_s=3.0)
         dt = 2e-4
     K = 4.0
-    print(f"features {scene}: {p.num_particles} particles", flush=True)
+    print(f"features {scene}: {p.num_particles} particles, neighbour lists {lists}", flush=True)
     with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, lists)
         f.step(dt, 30)
         f.sort_.run()
         f.runDensityProg()
@@ -153,7 +155,8 @@ def _zero_net():
     return net
 
 
-def test_zero_network_equals_plain_large_steps():
+@pytest.mark.parametrize("lists", [0, 1])
+def test_zero_network_equals_plain_large_steps(lists):
     p, parts, dt, _ = scenes.workload("dam_break_16k")
     K, macro = 4, 10
     with Fluid(p, parts) as a:
@@ -161,6 +164,7 @@ def test_zero_network_equals_plain_large_steps():
         want = a.particles(1)
     net = _zero_net()  # keep the handle alive for the whole run
     with Fluid(p, parts) as b:
+        b.set_tuning(2, 2, lists)
         st = L.wc_closure_stats()
         L.check(L.lib().wc_closure_run(b.ctx, net.h, C.c_float(dt), K, macro, C.c_float(1e9), C.byref(st)),
                 "wc_closure_run")
@@ -168,13 +172,15 @@ def test_zero_network_equals_plain_large_steps():
         assert b.particles(1).tobytes() == want.tobytes()
 
 
-def test_rejecting_judge_reproduces_plain_small_steps(oracle):
+@pytest.mark.parametrize("lists", [0, 1])
+def test_rejecting_judge_reproduces_plain_small_steps(lists, oracle):
     p, parts, dt, _ = scenes.workload("dam_break_16k")
     K, macro = 4, 6
     ref = oracle.OracleState(p, parts)
     ref.step(dt, K * macro)
     net = _zero_net()
     with Fluid(p, parts) as b:
+        b.set_tuning(2, 2, lists)
         st = L.wc_closure_stats()
         L.check(L.lib().wc_closure_run(b.ctx, net.h, C.c_float(dt), K, macro, C.c_float(-1.0), C.byref(st)),
                 "wc_closure_run")
@@ -194,7 +200,8 @@ def test_identities_follow_particles():
         assert np.array_equal(f.ids(2), ids[order])
 
 
-def test_neighbour_averaged_correction_of_a_constant_field():
+@pytest.mark.parametrize("lists", [0, 1])
+def test_neighbour_averaged_correction_of_a_constant_field(lists):
     """A network whose output is the constant c: averaged over neighbours, the correction of a
     particle whose neighbours all lie 2h or more from the walls is c again (up to rounding), while
     within 3h of a wall the neighbours there count with zero and pull it down."""
@@ -209,6 +216,7 @@ def test_neighbour_averaged_correction_of_a_constant_field():
     out = {}
     for smooth in (0, 1):
         with Fluid(p, parts) as f:
+            f.set_tuning(2, 2, lists)
             L.check(L.lib().wc_closure_set_smoothing(f.ctx, smooth), "smoothing")
             st = L.wc_closure_stats()
             L.check(L.lib().wc_closure_run(f.ctx, net.h, C.c_float(dt), K, 1, C.c_float(1e30), C.byref(st)),

@@ -1220,7 +1220,8 @@ WC_API int wc_closure_features(wc_ctx* c, float dt_mult, const uint32_t* rows_ho
     } else {
         nrows = n;
     }
-    WC_HIP(wc::launch_closure_features(c->b, rows, nrows, dt_mult, 1.0f, (uint16_t*)x_dev, c->stream));
+    WC_HIP(wc::launch_closure_features(c->b, rows, nrows, dt_mult, 1.0f, (uint16_t*)x_dev, c->neighbour_list,
+                                       c->stream));
     WC_HIP(hipStreamSynchronize(c->stream));
     return WC_OK;
 }
@@ -1231,7 +1232,7 @@ WC_API int wc_closure_update(wc_ctx* c, float dt, const void* corr_dev, float co
     int rc = set_device(c);
     if (rc) return rc;
     WC_HIP(wc::launch_closure_update(c->b, dt, (const float*)corr_dev, corr_scale, (float*)acc_dev, false, 0.0f,
-                                     c->stream));
+                                     c->neighbour_list, c->stream));
     return WC_OK;
 }
 
@@ -1330,13 +1331,14 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
             e = hipMemcpyAsync(c->cl_backup_ids, c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream);
         if (net) {
             if (e == hipSuccess)
-                e = wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->stream);
+                e = wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->neighbour_list,
+                                                c->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(net->m.stream);
             if (e == hipSuccess) e = wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream);
         }
         if (e == hipSuccess)
             e = wc::launch_closure_update(c->b, dt_large, net ? c->cl_y : nullptr, g, nullptr, c->cl_smooth,
-                                          c->cl_substeps ? (float)K : 0.0f, c->stream);
+                                          c->cl_substeps ? (float)K : 0.0f, c->neighbour_list, c->stream);
         if (e != hipSuccess) rc = fail(WC_ERR_HIP, "learned step: %s", hipGetErrorString(e));
         stats->macro_steps += 1;
     }

@@ -104,15 +104,17 @@ struct DeviceBuffers {
 // Learned closure (single-scene contexts).
 constexpr int kClosureNeighbours = 32;
 constexpr int kClosureFeatures = 208;  // 195 used, zero padded
-// Features of the sorted particles at `rows` (slots; NULL = every slot) into X (bf16 rows).
+// Features of the sorted particles at `rows` (slots; NULL = every slot) into X (bf16 rows); with
+// `lists`, from the neighbour lists of the last density pass instead of a search.
 hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows, uint32_t nrows, float dt_mult,
-                                   float v_ref, uint16_t* X, hipStream_t stream);
+                                   float v_ref, uint16_t* X, bool lists, hipStream_t stream);
 // Force pass with dt given directly; corr (4 floats per slot, units of g, may be NULL) is added to
 // the acceleration of particles at least 2h from every wall, averaged over each particle's
 // neighbours when `smooth`; acc_out (may be NULL) receives the plain acceleration.  substeps = K
 // > 0 integrates as K steps of dt / K under the frozen force (0: one symplectic Euler step).
+// With `lists` the neighbours come from the last density pass's neighbour lists.
 hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
-                                 float* acc_out, bool smooth, float substeps, hipStream_t stream);
+                                 float* acc_out, bool smooth, float substeps, bool lists, hipStream_t stream);
 
 // Tuning variant of the neighbour passes; all variants produce identical results.
 struct PassVariant {

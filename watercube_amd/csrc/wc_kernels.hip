@@ -714,10 +714,22 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                 const uint32_t u0 = nlist[t];
                 uint32_t u1 = nlist[(size_t)(last < 1u ? last : 1u) * stride + t];
                 float4 q = posrho_in[u0], qv = velp_in[u0];
+                uint32_t u = u0;
                 for (uint32_t k = 0; k < cnt; ++k) {
                     const uint32_t u2 = nlist[(size_t)(k + 2 < cnt ? k + 2 : last) * stride + t];
                     const float4 qn = posrho_in[u1], qvn = velp_in[u1];
-                    f.add(c, p, pv, q, qv);
+                    const float dist = f.add(c, p, pv, q, qv);
+                    if constexpr (MODE == 2) {
+                        const float w = c.h - dist;
+                        if (corr && closure_interior(c, q.x, q.y, q.z)) {
+                            const float4 cj = corr[u];
+                            csum.x += w * cj.x;
+                            csum.y += w * cj.y;
+                            csum.z += w * cj.z;
+                        }
+                        wsum += w;
+                    }
+                    u = u1;
                     q = qn;
                     qv = qvn;
                     u1 = u2;
@@ -818,7 +830,9 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
                                                                  const uint32_t* __restrict__ skey,
                                                                  const uint2* __restrict__ range,
                                                                  const uint32_t* __restrict__ rows, uint32_t nrows,
-                                                                 float dt_mult, float inv_v, uint16_t* __restrict__ X) {
+                                                                 float dt_mult, float inv_v, uint16_t* __restrict__ X,
+                                                                 const uint32_t* __restrict__ nlist,
+                                                                 const uint32_t* __restrict__ ncount) {
     static_assert(kFeatChunk <= (int)kRecordPad + 1, "chunked loads may run kFeatChunk - 1 records past a cell");
     static_assert(kFeatCap >= kClosureNeighbours + kFeatChunk, "a chunk must fit after a selection");
     __shared__ uint32_t s_key[kFeatCap * kFeatBlock];
@@ -859,7 +873,31 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
         }
     };
     auto nothing = []() {};
-    for_each_neighbour_cell(skey[t], range + d.c_off, c, d.p_off + c.n, visit, nothing);
+    // With the density pass's neighbour lists the accepted neighbours are read in the same order
+    // instead of searched for (a list that overflowed is searched for as without lists).
+    const uint32_t listed = ncount ? ncount[t] : kListOverflow;
+    if (listed != kListOverflow) {
+        const size_t stride = S.total_particles;
+        for (uint32_t k = 0; k < listed; k += kFeatChunk) {
+            if (cnt > kFeatCap - kFeatChunk) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
+            uint32_t u[kFeatChunk];
+            float4 v[kFeatChunk];
+#pragma unroll
+            for (int j = 0; j < kFeatChunk; ++j) u[j] = nlist[(size_t)min(k + (uint32_t)j, listed - 1) * stride + t];
+#pragma unroll
+            for (int j = 0; j < kFeatChunk; ++j) v[j] = posrho[u[j]];
+#pragma unroll
+            for (int j = 0; j < kFeatChunk; ++j) {
+                const float rx = p.x - v[j].x, ry = p.y - v[j].y, rz = p.z - v[j].z;
+                const float d2 = (rx * rx + ry * ry) + rz * rz;
+                key[cnt * kFeatBlock] = __float_as_uint(d2);
+                slot[cnt * kFeatBlock] = u[j];
+                cnt += (k + (uint32_t)j < listed) ? 1 : 0;
+            }
+        }
+    } else {
+        for_each_neighbour_cell(skey[t], range + d.c_off, c, d.p_off + c.n, visit, nothing);
+    }
     if (cnt > kClosureNeighbours) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
 
     const float inv_h = 1.0f / c.h;
@@ -1024,16 +1062,32 @@ hipError_t launch_update(const DeviceBuffers& b, int v, bool list, hipStream_t s
 }
 
 hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows, uint32_t nrows, float dt_mult,
-                                   float v_ref, uint16_t* X, hipStream_t s) {
+                                   float v_ref, uint16_t* X, bool lists, hipStream_t s) {
     if (nrows == 0) return hipSuccess;
+    if (lists && (!b.nlist || !b.ncount)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(nrows, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc,
-                       b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, 1.0f / v_ref, X);
+                       b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, 1.0f / v_ref, X,
+                       lists ? b.nlist : nullptr, lists ? b.ncount : nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
-                                 float* acc_out, bool smooth, float substeps, hipStream_t s) {
+                                 float* acc_out, bool smooth, float substeps, bool lists, hipStream_t s) {
     if (b.sc.blocks64 == 0) return hipSuccess;
+    if (lists && (!b.nlist || !b.ncount)) return hipErrorInvalidValue;
+    if (lists) {
+        if (smooth)
+            hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, true>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
+                               b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done,
+                               b.ids[1], b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
+                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps);
+        else
+            hipLaunchKernelGGL((k_update<64, 32, 4, true, 1, true>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
+                               b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done,
+                               b.ids[1], b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
+                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps);
+        return hipGetLastError();
+    }
     if (smooth)
         hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
