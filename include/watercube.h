@@ -164,7 +164,10 @@ WC_API int wc_density(wc_ctx* ctx);
 /* Fluid::runUpdateProg(buffer2, buffer1, dt) (src/core/Fluid.cpp:294-321): update.comp:134-232.
  * `dt` is the integration step itself (the reference passes frame time * time_scale_).  Reads
  * the densities and pressures of the preceding wc_density on the same sort, as the reference's
- * pass order does. */
+ * pass order does.  The sort and the density pass also check that every position, velocity,
+ * density and pressure is 0 or within [2^-40, 2^40] in magnitude; when they are, the pass forms
+ * each neighbour's quotients from shared reciprocals (same bits as IEEE division), otherwise,
+ * and when no density pass followed the last sort, it divides as IEEE does. */
 WC_API int wc_update(wc_ctx* ctx, float dt);
 
 /* nsteps of Fluid::update (src/core/Fluid.cpp:342-354): sort, density, update, replayed as one
@@ -209,7 +212,9 @@ WC_API int wc_get_sorted(wc_ctx* ctx, uint32_t* host, uint32_t n);
 
 /* Device addresses of the context's buffers, for callers that keep data in HBM.
  * posrho[b] / velp[b] are float4 arrays (x, y, z, density) and (vx, vy, vz, pressure) of
- * particle buffer b+1; cell_range is uint2 (offset, count) per cell. */
+ * particle buffer b+1; cell_range is uint2 (offset, count) per cell.  Buffer 2 is the sort's
+ * output: a caller that changes particles writes buffer 1 and sorts (wc_sort_run or wc_step)
+ * before the next density and force pass, which rely on the sort's checks. */
 typedef struct wc_device_view {
     void* posrho[2];
     void* velp[2];

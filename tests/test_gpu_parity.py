@@ -404,3 +404,47 @@ def test_graph_replays_interleaved_with_eager_work(oracle):
         f.step(dt, 3)
         st.step(dt, 3)
         _check_records(f.particles(1), st.buf1, "three more replays")
+
+
+@pytest.mark.parametrize("lists", [0, 1])
+def test_force_pass_division_modes(lists, oracle):
+    """The force pass shares one reciprocal between a neighbour's quotients only when the density
+    pass found every operand 0 or within [2^-40, 2^40]; otherwise it divides as IEEE does.  Both
+    paths must give the oracle's bits: an in-range scene with velocity components of -0 (whose
+    quotients' zero sign differs between the paths), and the same scene with one velocity
+    component of 1e-30, which switches the whole step to IEEE division."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    neg0 = parts.copy()
+    neg0["velocity"][::2, 0] = np.float32(-0.0)
+    neg0["velocity"][1::3, 2] = np.float32(-0.0)
+    tiny = neg0.copy()
+    tiny["velocity"][100, 1] = np.float32(1e-30)
+    for what, state in (("-0 velocities", neg0), ("a 1e-30 velocity", tiny)):
+        _log(f"division modes: {what}, lists {lists}")
+        st = oracle.OracleState(p, state)
+        st.step(dt, 20)
+        with Fluid(p, state) as f:
+            f.set_tuning(2, 2, lists)
+            f.step(dt, 20)
+            _check_records(f.particles(1), st.buf1, f"{what}, lists {lists}")
+
+
+def test_force_pass_right_after_a_sort(oracle):
+    """A force pass straight after a sort, with no density pass between them (the densities are
+    then the ones the records carried in): it must not rely on the last density pass's operand
+    check, which describes an older sorted buffer, and so divides as IEEE does."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 3)
+    with Fluid(p, parts) as f:
+        f.step(dt, 3)  # leaves a density pass's state behind
+        # move the particles, then sort and update only
+        moved = f.particles(1)
+        moved["position"][:, 0] += np.float32(0.001)
+        stm = oracle.OracleState(p, moved)
+        f.upload(moved)
+        f.sort_.run()
+        f.runUpdateProg(dt)
+        stm.sort_run()
+        stm.update(dt)
+        _check_records(f.particles(1), stm.buf1, "update after sort without density")

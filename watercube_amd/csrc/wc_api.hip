@@ -355,6 +355,8 @@ int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
     zero(b.scene_done, (size_t)num * 8 + 8);
     zero(b.step_start, 64);
     zero(b.rho_max, 64);
+    // rho_max[1] = 2: no density pass has checked the force pass's division operands yet
+    if (e == hipSuccess) e = hipMemsetAsync(reinterpret_cast<char*>(b.rho_max) + 4, 2, 1, c->stream);
     zero(b.sorted, n * 4);
     zero(b.skey, n * 4);
     zero(b.key, n * 4);

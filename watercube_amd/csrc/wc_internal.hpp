@@ -94,7 +94,8 @@ struct DeviceBuffers {
     uint32_t* error_flag;             // nonzero after a failed bounded wait
     unsigned long long* scene_done;   // per scene: wall clock when its force pass finished
     unsigned long long* step_start;   // wall clock when the last step's hash pass started
-    uint32_t* rho_max;                // bits of the largest density of the last density pass
+    uint32_t* rho_max;                // [0] bits of the largest density of the last density pass;
+                                      // [1] 0 when the force pass may share reciprocals
     uint32_t* ids[2];                 // particle identity per slot of buffer 1 / 2 (NULL: not tracked)
     uint32_t* nlist;                  // neighbour lists (NULL until the list mode is first used)
     uint32_t* ncount;

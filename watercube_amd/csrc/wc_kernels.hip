@@ -68,6 +68,10 @@ __device__ __forceinline__ uint32_t total_blocks(const Scenes& S) {
     return BLOCK == 64 ? S.blocks64 : (BLOCK == 128 ? S.blocks128 : S.blocks256);
 }
 
+// Bits of rho_max[1], the force pass's division mode: 0 lets it share reciprocals (quot).
+constexpr uint32_t kDivUnsafe = 1u;   // an operand of this step is outside div_operand's range
+constexpr uint32_t kDivPending = 2u;  // sorted since the last density pass
+
 // Walks the in-grid neighbour cells of scene-local cell `key` in NEIGHBORHOOD order
 // (density.comp:40-50: dx outermost, dz innermost); visit(start, end, own) receives each cell's
 // slot range (clipped to `limit`) and whether it is the particle's own cell (the 14th of the 27,
@@ -119,7 +123,8 @@ __global__ __launch_bounds__(kHashBlock) void k_hash(Scenes S, const float4* __r
     if (g == 0) {
         *scan_ticket = 0u;
         *step_start = wall_clock64();
-        *rho_max = 0u;
+        rho_max[0] = 0u;
+        rho_max[1] = kDivPending;  // the force pass's operands are unchecked until a density pass runs
     }
     const uint32_t b = blockIdx.x;
     bool valid = false;
@@ -309,6 +314,25 @@ __global__ __launch_bounds__(kHashBlock) void k_place(Scenes S, const uint32_t* 
     if (slot - d.p_off < d.c.n) slot_tmp[slot] = i;  // always true when the scan succeeded
 }
 
+// Correctly rounded a / b from y = RN(1 / b): q0 = RN(a y), r = a - b q0 (exact), RN(q0 + r y)
+// (Markstein's theorem).  Equal to the IEEE quotient, except that a = -0 gives +0, when b and y
+// are normal and a is 0 or at least 2^-100 in magnitude with a normal quotient, so that no step
+// underflows or overflows.  The force pass uses it only on steps whose operands the density pass
+// found in that range (div_operand); then three quotients share one division.
+__device__ __forceinline__ float quot(float a, float b, float y) {
+    const float q0 = a * y;
+    const float r = fmaf(-b, q0, a);
+    return fmaf(r, y, q0);
+}
+
+// True for 0 and for 2^-40 <= |x| <= 2^40 (false for NaN and infinity).  Sums and differences of
+// two such values are 0 or at least 2^-63 and at most 2^41 in magnitude, which keeps every
+// quotient of the force pass inside the range where quot() is exact.
+__device__ __forceinline__ bool div_operand(float x) {
+    const uint32_t u = __float_as_uint(x) & 0x7fffffffu;
+    return u == 0u || u - 0x2b800000u <= 0x53800000u - 0x2b800000u;
+}
+
 // ------------------------------------------------------------------------------------------
 // 4. stable rank + gather: the particle that arrived at slot t moves to the cell's first slot +
 //    (number of particles of its cell with a smaller input index), so every cell keeps input
@@ -321,7 +345,8 @@ __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t*
                                                        const float4* __restrict__ velp_in,
                                                        float4* __restrict__ posrho_out, float4* __restrict__ velp_out,
                                                        uint32_t* __restrict__ sorted, uint32_t* __restrict__ skey,
-                                                       const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id_out) {
+                                                       const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id_out,
+                                                       uint32_t* __restrict__ div_flags) {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
@@ -340,9 +365,12 @@ __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t*
     if (dst >= lim) return;
     sorted[dst] = i;
     skey[dst] = k;
+    const float4 v = velp_in[i];
     posrho_out[dst] = posrho_in[i];
-    velp_out[dst] = velp_in[i];
+    velp_out[dst] = v;
     if (id_out) id_out[dst] = id_in[i];
+    // velocities are operands of the force pass's quotients (the density pass checks the rest)
+    if (!(div_operand(v.x) && div_operand(v.y) && div_operand(v.z))) atomicOr(div_flags, kDivUnsafe);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -441,47 +469,58 @@ __global__ __launch_bounds__(BLOCK) void k_density(Scenes S, float4* __restrict_
                                                    uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount) {
     using Entry = typename std::conditional<LIST, uint32_t, float>::type;
     __shared__ Entry s_list[CAP * BLOCK];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(&rho_max[1], ~kDivPending);
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
     const StepConsts& c = d.c;
     const uint32_t lt = (b - first_blocks<BLOCK>(S)[s]) * BLOCK + threadIdx.x;
-    if (lt >= c.n) return;
-    const uint32_t t = d.p_off + lt;
-    const float4 p = posrho[t];
-    const lds_ptr<Entry> list = (lds_ptr<Entry>)s_list + threadIdx.x;
-    float rho = c.m * poly6(c, 0.0f);
-    uint32_t written = 0;  // LIST: neighbours found so far
-    auto drain = [&](int cnt) {
-        if constexpr (LIST) {
-            if (cnt <= 0) return;
-            uint32_t u = list[0];
-            float4 q = posrho[u];
-            for (int k = 0; k < cnt; ++k) {
-                const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
-                const float4 qn = posrho[un];
-                const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-                const float d2 = (rx * rx + ry * ry) + rz * rz;
-                rho += c.m * poly6(c, sqrtf(d2));
-                if (written < kNeighbourListMax) nlist[(size_t)written * S.total_particles + t] = u;
-                ++written;
-                u = un;
-                q = qn;
+    if (lt - __lane_id() >= c.n) return;  // the whole wave is past the scene's last particle
+    const bool active = lt < c.n;
+    float density = 0.0f;
+    if (active) {
+        const uint32_t t = d.p_off + lt;
+        const float4 p = posrho[t];
+        const lds_ptr<Entry> list = (lds_ptr<Entry>)s_list + threadIdx.x;
+        float rho = c.m * poly6(c, 0.0f);
+        uint32_t written = 0;  // LIST: neighbours found so far
+        auto drain = [&](int cnt) {
+            if constexpr (LIST) {
+                if (cnt <= 0) return;
+                uint32_t u = list[0];
+                float4 q = posrho[u];
+                for (int k = 0; k < cnt; ++k) {
+                    const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
+                    const float4 qn = posrho[un];
+                    const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                    const float d2 = (rx * rx + ry * ry) + rz * rz;
+                    rho += c.m * poly6(c, sqrtf(d2));
+                    if (written < kNeighbourListMax) nlist[(size_t)written * S.total_particles + t] = u;
+                    ++written;
+                    u = un;
+                    q = qn;
+                }
+            } else {
+                for (int k = 0; k < cnt; ++k) rho += c.m * poly6(c, sqrtf(list[k * BLOCK]));
             }
-        } else {
-            for (int k = 0; k < cnt; ++k) rho += c.m * poly6(c, sqrtf(list[k * BLOCK]));
-        }
-    };
-    search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho,
-                                                      range + d.c_off, c, list, drain);
-    if constexpr (LIST) ncount[t] = written <= kNeighbourListMax ? written : kListOverflow;
-    const float wall = c.wall_density ? wall_density(c, p.x, p.y, p.z) : 0.0f;
-    const float density = rho + wall;
-    reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = density;
-    reinterpret_cast<float*>(velp)[4 * (size_t)t + 3] = eos(c, rho);
-    // largest density of the pass (positive floats order as their bit patterns; NaN ranks highest)
-    uint32_t bits = __float_as_uint(density) & 0x7fffffffu;
-    for (int d = 32; d >= 1; d >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, d));
+        };
+        search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho,
+                                                          range + d.c_off, c, list, drain);
+        if constexpr (LIST) ncount[t] = written <= kNeighbourListMax ? written : kListOverflow;
+        const float wall = c.wall_density ? wall_density(c, p.x, p.y, p.z) : 0.0f;
+        density = rho + wall;
+        const float pressure = eos(c, rho);
+        reinterpret_cast<float*>(posrho)[4 * (size_t)t + 3] = density;
+        reinterpret_cast<float*>(velp)[4 * (size_t)t + 3] = pressure;
+        // the other operands of the force pass's quotients (the sort checks the velocities)
+        const bool ok = div_operand(p.x) && div_operand(p.y) && div_operand(p.z) && div_operand(pressure) &&
+                        div_operand(density) && density != 0.0f;
+        if (!ok) atomicOr(&rho_max[1], kDivUnsafe);
+    }
+    // largest density of the pass (positive floats order as their bit patterns; NaN ranks highest);
+    // lanes past the scene's last particle take part as 0
+    uint32_t bits = active ? __float_as_uint(density) & 0x7fffffffu : 0u;
+    for (int k = 32; k >= 1; k >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, k));
     if ((threadIdx.x & 63) == 0) atomicMax(rho_max, bits);
 }
 
@@ -492,9 +531,14 @@ struct V3 {
     float x, y, z;
 };
 
+template <bool FAST = false>
 __device__ __forceinline__ V3 spiky(const StepConsts& c, float rx, float ry, float rz, float d) {
     const float a = c.h - d;
     const float s = a * a;
+    if constexpr (FAST) {
+        const float y = 1.0f / d;
+        return V3{(s * quot(rx, d, y)) * c.spiky, (s * quot(ry, d, y)) * c.spiky, (s * quot(rz, d, y)) * c.spiky};
+    }
     return V3{(s * (rx / d)) * c.spiky, (s * (ry / d)) * c.spiky, (s * (rz / d)) * c.spiky};
 }
 
@@ -571,23 +615,39 @@ __device__ __forceinline__ void integrate_axis(const StepConsts& c, float pf, fl
 struct ForceSums {
     float pfx, pfy, pfz, vfx, vfy, vfz;
 
-    // returns the pair distance
+    // Returns the pair distance.  FAST: the quotients by rho_j, by 2 rho_j and by the distance
+    // come from one reciprocal each (quot); the results are the same bits as with FAST false
+    // whenever every operand passes div_operand.  A zero quotient may then carry the other sign,
+    // which these sums cannot show: they start at +0, and x + (+-0) = x, +0 - (+-0) = +0.
+    template <bool FAST>
     __device__ __forceinline__ float add(const StepConsts& c, const float4& p, const float4& pv, const float4& q,
                                          const float4& qv) {
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float dist = sqrtf((rx * rx + ry * ry) + rz * rz);
-        const float pressure = (pv.w + qv.w) / (2.0f * q.w);
+        float iw = 0.0f, pressure;
+        if constexpr (FAST) {
+            iw = 1.0f / q.w;  // RN(1 / rho_j); RN(1 / (2 rho_j)) is half of it
+            pressure = quot(pv.w + qv.w, 2.0f * q.w, iw * 0.5f);
+        } else {
+            pressure = (pv.w + qv.w) / (2.0f * q.w);
+        }
         if (pressure > 0.0f) {
-            const V3 sw = spiky(c, rx, ry, rz, dist + 1e-16f);
+            const V3 sw = spiky<FAST>(c, rx, ry, rz, dist + 1e-16f);
             const float mp = c.m * pressure;
             pfx = pfx - mp * sw.x;
             pfy = pfy - mp * sw.y;
             pfz = pfz - mp * sw.z;
         }
         const float wv = (c.h - dist) * c.visc;
-        vfx = vfx + (c.m * ((qv.x - pv.x) / q.w)) * wv;
-        vfy = vfy + (c.m * ((qv.y - pv.y) / q.w)) * wv;
-        vfz = vfz + (c.m * ((qv.z - pv.z) / q.w)) * wv;
+        if constexpr (FAST) {
+            vfx = vfx + (c.m * quot(qv.x - pv.x, q.w, iw)) * wv;
+            vfy = vfy + (c.m * quot(qv.y - pv.y, q.w, iw)) * wv;
+            vfz = vfz + (c.m * quot(qv.z - pv.z, q.w, iw)) * wv;
+        } else {
+            vfx = vfx + (c.m * ((qv.x - pv.x) / q.w)) * wv;
+            vfy = vfy + (c.m * ((qv.y - pv.y) / q.w)) * wv;
+            vfz = vfz + (c.m * ((qv.z - pv.z) / q.w)) * wv;
+        }
         return dist;
     }
 };
@@ -661,7 +721,8 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                                                   const uint32_t* __restrict__ id_sorted, uint32_t* __restrict__ id_out,
                                                   float dt_arg, const float4* __restrict__ corr, float corr_scale,
                                                   float4* __restrict__ acc_out, const uint32_t* __restrict__ nlist,
-                                                  const uint32_t* __restrict__ ncount, float substeps) {
+                                                  const uint32_t* __restrict__ ncount, float substeps,
+                                                  const uint32_t* __restrict__ div_flags) {
     __shared__ uint32_t s_list[CAP * BLOCK];
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
@@ -676,70 +737,89 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
     ForceSums f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     V3 csum{0.0f, 0.0f, 0.0f};  // MODE 2: weighted sum of neighbour corrections
     float wsum = 0.0f;
-    // neighbour k + 1's records are loaded while neighbour k's terms are formed
-    auto drain = [&](int cnt) {
-        if (cnt <= 0) return;
-        uint32_t u = list[0];
-        float4 q = posrho_in[u], qv = velp_in[u];
-        for (int k = 0; k < cnt; ++k) {
-            const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
-            const float4 qn = posrho_in[un], qvn = velp_in[un];
-            const float dist = f.add(c, p, pv, q, qv);
-            if constexpr (MODE == 2) {
-                const float w = c.h - dist;
-                if (corr && closure_interior(c, q.x, q.y, q.z)) {
-                    const float4 cj = corr[u];
-                    csum.x += w * cj.x;
-                    csum.y += w * cj.y;
-                    csum.z += w * cj.z;
-                }
-                wsum += w;
-            }
-            u = un;
-            q = qn;
-            qv = qvn;
-        }
-    };
-    bool searched = true;
-    if constexpr (LIST) {
-        const uint32_t cnt = ncount[t];
-        if (cnt != kListOverflow) {
-            searched = false;
-            // the same loop as the drain, over the stored list (entry k at nlist[k * total + t]):
-            // entry k + 2's index and entry k + 1's records load while entry k's terms are formed,
-            // so no gather waits for the index it needs
-            const size_t stride = S.total_particles;
-            if (cnt > 0) {
-                const uint32_t last = cnt - 1;
-                const uint32_t u0 = nlist[t];
-                uint32_t u1 = nlist[(size_t)(last < 1u ? last : 1u) * stride + t];
-                float4 q = posrho_in[u0], qv = velp_in[u0];
-                uint32_t u = u0;
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    const uint32_t u2 = nlist[(size_t)(k + 2 < cnt ? k + 2 : last) * stride + t];
-                    const float4 qn = posrho_in[u1], qvn = velp_in[u1];
-                    const float dist = f.add(c, p, pv, q, qv);
-                    if constexpr (MODE == 2) {
-                        const float w = c.h - dist;
-                        if (corr && closure_interior(c, q.x, q.y, q.z)) {
-                            const float4 cj = corr[u];
-                            csum.x += w * cj.x;
-                            csum.y += w * cj.y;
-                            csum.z += w * cj.z;
-                        }
-                        wsum += w;
+    // Every neighbour's pressure and viscosity terms.  FAST (a density pass ran since the last sort
+    // and found all of this step's operands in div_operand's range; a wave-uniform flag) shares
+    // one reciprocal between the quotients of a neighbour; the results are the same bits either
+    // way.
+    auto accumulate = [&](auto fast) {
+        constexpr bool FAST = decltype(fast)::value;
+        auto load = [&](uint32_t u, float4& q, float4& qv) {
+            q = posrho_in[u];
+            qv = velp_in[u];
+        };
+        // neighbour k + 1's records are loaded while neighbour k's terms are formed
+        auto drain = [&](int cnt) {
+            if (cnt <= 0) return;
+            uint32_t u = list[0];
+            float4 q, qv;
+            load(u, q, qv);
+            for (int k = 0; k < cnt; ++k) {
+                const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
+                float4 qn, qvn;
+                load(un, qn, qvn);
+                const float dist = f.template add<FAST>(c, p, pv, q, qv);
+                if constexpr (MODE == 2) {
+                    const float w = c.h - dist;
+                    if (corr && closure_interior(c, q.x, q.y, q.z)) {
+                        const float4 cj = corr[u];
+                        csum.x += w * cj.x;
+                        csum.y += w * cj.y;
+                        csum.z += w * cj.z;
                     }
-                    u = u1;
-                    q = qn;
-                    qv = qvn;
-                    u1 = u2;
+                    wsum += w;
+                }
+                u = un;
+                q = qn;
+                qv = qvn;
+            }
+        };
+        bool searched = true;
+        if constexpr (LIST) {
+            const uint32_t cnt = ncount[t];
+            if (cnt != kListOverflow) {
+                searched = false;
+                // the same loop as the drain, over the stored list (entry k at nlist[k * total + t]):
+                // entry k + 2's index and entry k + 1's records load while entry k's terms are formed,
+                // so no gather waits for the index it needs
+                const size_t stride = S.total_particles;
+                if (cnt > 0) {
+                    const uint32_t last = cnt - 1;
+                    const uint32_t u0 = nlist[t];
+                    uint32_t u1 = nlist[(size_t)(last < 1u ? last : 1u) * stride + t];
+                    float4 q, qv;
+                    load(u0, q, qv);
+                    uint32_t u = u0;
+                    for (uint32_t k = 0; k < cnt; ++k) {
+                        const uint32_t u2 = nlist[(size_t)(k + 2 < cnt ? k + 2 : last) * stride + t];
+                        float4 qn, qvn;
+                        load(u1, qn, qvn);
+                        const float dist = f.template add<FAST>(c, p, pv, q, qv);
+                        if constexpr (MODE == 2) {
+                            const float w = c.h - dist;
+                            if (corr && closure_interior(c, q.x, q.y, q.z)) {
+                                const float4 cj = corr[u];
+                                csum.x += w * cj.x;
+                                csum.y += w * cj.y;
+                                csum.z += w * cj.z;
+                            }
+                            wsum += w;
+                        }
+                        u = u1;
+                        q = qn;
+                        qv = qvn;
+                        u1 = u2;
+                    }
                 }
             }
         }
-    }
-    if (searched)
-        search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho_in,
-                                                          range + d.c_off, c, list, drain);
+        if (searched)
+            search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho_in,
+                                                              range + d.c_off, c, list, drain);
+    };
+    if (div_flags[0] == 0u)
+        accumulate(std::true_type{});
+    else
+        accumulate(std::false_type{});
 
     V3 ext{c.g[0] * p.w, c.g[1] * p.w, c.g[2] * p.w};
     V3 mouse{0.0f, 0.0f, 0.0f}, wall{0.0f, 0.0f, 0.0f};
@@ -985,7 +1065,8 @@ hipError_t launch_place(const DeviceBuffers& b, hipStream_t s) {
 hipError_t launch_gather(const DeviceBuffers& b, hipStream_t s) {
     if (b.sc.blocks256 == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, dim3(b.sc.blocks256), dim3(kHashBlock), 0, s, b.sc, b.slot_tmp, b.key, b.range,
-                       b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, b.ids[0], b.ids[1]);
+                       b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, b.ids[0], b.ids[1],
+                       b.rho_max + 1);
     return hipGetLastError();
 }
 
@@ -1048,12 +1129,12 @@ hipError_t launch_update(const DeviceBuffers& b, int v, bool list, hipStream_t s
             hipLaunchKernelGGL((k_update<B, C, K, P, 0, true>), dim3(blocks), dim3(B), 0, s, b.sc,         \
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
                                b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, b.nlist,    \
-                               b.ncount, 0.0f);                                                                 \
+                               b.ncount, 0.0f, b.rho_max + 1);                                                  \
         else                                                                                              \
             hipLaunchKernelGGL((k_update<B, C, K, P, 0, false>), dim3(blocks), dim3(B), 0, s, b.sc,        \
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
                                b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, nullptr,    \
-                               nullptr, 0.0f);                                                                  \
+                               nullptr, 0.0f, b.rho_max + 1);                                                   \
         return hipGetLastError();                                                                         \
     }
     WC_VARIANTS(WC_LAUNCH)
@@ -1080,24 +1161,24 @@ hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* 
             hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, true>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done,
                                b.ids[1], b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps);
+                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps, b.rho_max + 1);
         else
             hipLaunchKernelGGL((k_update<64, 32, 4, true, 1, true>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done,
                                b.ids[1], b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps);
+                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps, b.rho_max + 1);
         return hipGetLastError();
     }
     if (smooth)
         hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps);
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1);
     else
         hipLaunchKernelGGL((k_update<64, 32, 4, true, 1, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps);
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1);
     return hipGetLastError();
 }
 

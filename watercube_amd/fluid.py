@@ -106,6 +106,11 @@ class Fluid:
     def synchronize(self) -> None:
         L.check(L.lib().wc_synchronize(self.ctx), "wc_synchronize")
 
+    def upload(self, particles: np.ndarray) -> None:
+        """Replace buffer 1 with `particles` (structured 32-byte records)."""
+        parts = np.ascontiguousarray(particles, dtype=L.PARTICLE_DTYPE)
+        L.check(L.lib().wc_upload(self.ctx, parts.ctypes.data, parts.shape[0]), "wc_upload")
+
     def particles(self, which: int = 1) -> np.ndarray:
         n = self.numParticles()
         out = np.empty(n, L.PARTICLE_DTYPE)
