@@ -448,3 +448,39 @@ def test_force_pass_right_after_a_sort(oracle):
         stm.sort_run()
         stm.update(dt)
         _check_records(f.particles(1), stm.buf1, "update after sort without density")
+
+
+PARAM_CASES = {
+    # densities near 1e-13, below the 2^-40 operand range: the force pass divides as IEEE does
+    "tiny mass": dict(particle_mass=1e-16),
+    # every pressure 0: the pressure term never applies, quotients of 0
+    "zero pressure": dict(stiffness=0.0, rest_pressure=0.0),
+    "negative rest pressure": dict(rest_pressure=-50.0),
+    "no viscosity": dict(viscosity=0.0),
+    "linear eos, stiff": dict(eos=L.WC_EOS_LINEAR, stiffness=1e6),
+    # pressures above 2^40: IEEE division again
+    "huge stiffness": dict(stiffness=1e13),
+    "no gravity, no walls": dict(gravity=(0.0, 0.0, 0.0), wall_density=0, wall_force=0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(PARAM_CASES))
+def test_parameter_extremes_bit_exact(case, oracle):
+    """Unusual constants move densities, pressures and forces across the force pass's division
+    guard in both directions; every combination must still give the oracle's bits, with and
+    without neighbour lists."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    for k, v in PARAM_CASES[case].items():
+        if isinstance(v, tuple):
+            for i, x in enumerate(v):
+                getattr(p, k)[i] = x
+        else:
+            setattr(p, k, v)
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 10)
+    for lists in (0, 1):
+        _log(f"parameter case {case}, lists {lists}")
+        with Fluid(p, parts) as f:
+            f.set_tuning(2, 2, lists)
+            f.step(dt, 10)
+            _check_records(f.particles(1), st.buf1, f"{case}, lists {lists}")
