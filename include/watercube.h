@@ -144,6 +144,13 @@ WC_API int wc_destroy(wc_ctx* ctx);
 WC_API int wc_set_params(wc_ctx* ctx, const wc_params* p);
 WC_API int wc_get_params(const wc_ctx* ctx, wc_params* out);
 
+/* Fluid::updateGravity (src/core/Fluid.cpp:259-263) with rotate_gravity_ on: the gravity
+ * direction in fluid coordinates, rotateWorldSpacePosition(vec3(0, -1, 0)) =
+ * transpose(toMat3(q)) * (0, -1, 0) (Fluid.cpp:239-242), for the fluid's orientation quaternion
+ * q = (w, x, y, z).  The caller scales it by the gravity strength into wc_params.gravity
+ * (Fluid.cpp:310).  Host arithmetic in float, as glm's. */
+WC_API int wc_gravity_direction(const float q_wxyz[4], float out[3]);
+
 /* Copy n records into particle buffer 1 (the buffer Fluid::update sorts from and the update
  * pass writes to: Fluid.cpp:146, 347-350).  Both buffers are filled, like Fluid.cpp:146 and 155.
  * n is the context's particle count (all scenes, concatenated, for a batch). */
@@ -169,6 +176,13 @@ WC_API int wc_density(wc_ctx* ctx);
  * each neighbour's quotients from shared reciprocals (same bits as IEEE division), otherwise,
  * and when no density pass followed the last sort, it divides as IEEE does. */
 WC_API int wc_update(wc_ctx* ctx, float dt);
+
+/* Fluid::runAdvectProg(buffer, time) (src/core/Fluid.cpp:326-336, advect.comp:20-59), whose call
+ * in Fluid::update is commented out (Fluid.cpp:351): a position-only step on buffer 1, in place.
+ * Per axis x = x + v dt, then if x < 0.01 or x > size - 0.01 it is clamped there and v = v * -0.3
+ * (the shader's own border and damping, not the parameter block's).  `dt` is the step itself
+ * (the reference passes time * time_scale_). */
+WC_API int wc_advect(wc_ctx* ctx, float dt);
 
 /* nsteps of Fluid::update (src/core/Fluid.cpp:342-354): sort, density, update, replayed as one
  * captured HIP graph per step.  Asynchronous: returns once the work is queued. */

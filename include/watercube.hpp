@@ -105,6 +105,13 @@ public:
         applyGravity();
         return *this;
     }
+    // Fluid::updateGravity (src/core/Fluid.cpp:259-263) with rotate_gravity_ on: the direction
+    // from the fluid's orientation q = (w, x, y, z) times the strength, pushed to a set-up context.
+    void updateGravity(const float q_wxyz[4]) {
+        check(wc_gravity_direction(q_wxyz, gravity_direction_), "wc_gravity_direction");
+        applyGravity();
+        if (ctx_) check(wc_set_params(ctx_, &params_), "wc_set_params");
+    }
     // Mouse ray in the fluid's frame (Fluid::setMouseRay, src/core/Fluid.h:125).
     void setMouseRay(const float origin[3], const float direction[3]) {
         for (int a = 0; a < 3; ++a) {
@@ -147,6 +154,8 @@ public:
     // The passes of one update, for callers that interleave their own work.
     void runDensityProg() { check(wc_density(ctx_), "wc_density"); }
     void runUpdateProg(float dt) { check(wc_update(ctx_, dt), "wc_update"); }
+    // Fluid::runAdvectProg (src/core/Fluid.cpp:326-336), on buffer 1; dt is the step itself
+    void runAdvectProg(float dt) { check(wc_advect(ctx_, dt), "wc_advect"); }
 
     void synchronize() { check(wc_synchronize(ctx_), "wc_synchronize"); }
 

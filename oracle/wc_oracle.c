@@ -14,6 +14,8 @@
  *   update.comp:48-232      -> wco_update   (file lines of the shader, not of this file)
  *   Sort::run  (src/core/Sort.cpp:254-267)     -> wco_sort_run
  *   Fluid::update (src/core/Fluid.cpp:342-354) -> wco_step
+ * and the sixth, advect.comp:20-59 (compiled by the reference but never dispatched), as
+ * wco_advect.
  *
  * GLSL leaves some arithmetic to the driver.  This restatement fixes it as follows, and the
  * GPU library reproduces the same choices:
@@ -39,6 +41,8 @@
  *                                zero viscosity; wall clamp and damping single-step outcomes;
  *                                O(N^2) brute-force force sum in NEIGHBORHOOD order
  *   wco_wall_density, wco_wall_forces, wco_mouse_force   hand-derived values at chosen points
+ *   wco_advect                   x + v dt per axis; clamp and damping single-step outcomes at
+ *                                the shader's border 0.01 and damping 0.3
  *   kernel constants             closed forms of src/core/Fluid.cpp:214-216
  */
 #include <math.h>
@@ -384,6 +388,27 @@ WCO_API void wco_update(const wc_params* P, const wc_particle* in, const uint32_
 #endif
     for (int64_t i = 0; i < n; ++i) update_one(P, in, counts, offsets, out, dt, (uint32_t)i);
     (void)nthreads;
+}
+
+/* advect.comp:20-59, in place: x = x + v dt, then per axis the clamp to [border, size - border]
+ * with v *= -damping, border 0.01 and damping 0.3 fixed in the shader (advect.comp:33-34). */
+WCO_API void wco_advect(const wc_params* P, wc_particle* ps, uint32_t n, float dt) {
+    const float damping = 0.3f, border = 0.01f;
+    const float hi = P->size - border;
+    for (uint32_t i = 0; i < n; ++i) {
+        float* x = ps[i].position;
+        float* v = ps[i].velocity;
+        for (int a = 0; a < 3; ++a) x[a] = x[a] + v[a] * dt;
+        for (int a = 0; a < 3; ++a) {
+            if (x[a] < border) {
+                v[a] = v[a] * -damping;
+                x[a] = border;
+            } else if (x[a] > hi) {
+                v[a] = v[a] * -damping;
+                x[a] = hi;
+            }
+        }
+    }
 }
 
 /* Fluid::update (Fluid.cpp:342-354) repeated nsteps times: sort buf1 -> buf2, density on buf2,

@@ -58,6 +58,7 @@ def lib():
             "wco_step": (None, [vp, vp, vp, u32p, u32p, u32p, u32p, C.c_float, C.c_uint32, C.c_int]),
             "wco_step_timed": (None, [vp, vp, vp, u32p, u32p, u32p, u32p, C.c_float, C.c_uint32, C.c_int,
                                       C.POINTER(C.c_double)]),
+            "wco_advect": (None, [vp, vp, C.c_uint32, C.c_float]),
             "wco_max_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -105,6 +106,10 @@ class OracleState:
     def update(self, dt: float, threads: int = 0):
         lib().wco_update(_pp(self.params), self.buf2.ctypes.data, _u32(self.counts), _u32(self.offsets),
                          self.buf1.ctypes.data, C.c_float(dt), threads or lib().wco_max_threads())
+
+    def advect(self, dt: float):
+        """advect.comp:20-59 on buffer 1, in place."""
+        lib().wco_advect(_pp(self.params), self.buf1.ctypes.data, self.buf1.shape[0], C.c_float(dt))
 
     def step(self, dt: float, nsteps: int = 1, threads: int = 0):
         lib().wco_step(_pp(self.params), self.buf1.ctypes.data, self.buf2.ctypes.data, _u32(self.keys),

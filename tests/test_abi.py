@@ -2,6 +2,7 @@
 declares, the ctypes mirror of every struct matches the C layout, presets and parameter
 derivation, the host-side scene generators, and the error paths of a GPU-less host."""
 import ctypes as C
+import math
 import re
 import subprocess
 import sys
@@ -204,3 +205,30 @@ def test_cutoff_threshold_equivalence():
         # every float in a window around t gives the same answer both ways
         xs = (t.view(np.uint32) + np.arange(-2000, 2000, dtype=np.int64)).astype(np.uint32).view(np.float32)
         assert np.array_equal(np.sqrt(xs) >= hf, xs >= t)
+
+
+def test_gravity_direction_closed_forms():
+    """Fluid::updateGravity (src/core/Fluid.cpp:259-263): transpose(toMat3(q)) * (0, -1, 0).
+    Closed forms for the identity, half turns and quarter turns, and agreement with a float64
+    rotation matrix built independently from the quaternion for random orientations."""
+    from watercube_amd.fluid import gravity_direction
+
+    s = math.sqrt(0.5)
+    assert gravity_direction((1, 0, 0, 0)).tolist() == [0.0, -1.0, 0.0]
+    assert gravity_direction((0, 1, 0, 0)).tolist() == [0.0, 1.0, 0.0]    # half turn about x
+    assert gravity_direction((0, 0, 0, 1)).tolist() == [0.0, 1.0, 0.0]    # half turn about z
+    assert gravity_direction((0, 0, 1, 0)).tolist() == [0.0, -1.0, 0.0]   # half turn about y: unchanged
+    g = gravity_direction((s, 0, 0, s))   # quarter turn about z: R^T maps -y to -x
+    assert np.allclose(g, [-1.0, 0.0, 0.0], atol=1e-6)
+    g = gravity_direction((s, s, 0, 0))   # quarter turn about x: R^T maps -y to +z
+    assert np.allclose(g, [0.0, 0.0, 1.0], atol=1e-6)
+    rng = np.random.default_rng(5)
+    for _ in range(100):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        want = R.T @ np.array([0.0, -1.0, 0.0])
+        assert np.allclose(gravity_direction(q.astype(np.float32)), want, atol=3e-6)

@@ -484,3 +484,23 @@ def test_parameter_extremes_bit_exact(case, oracle):
             f.set_tuning(2, 2, lists)
             f.step(dt, 10)
             _check_records(f.particles(1), st.buf1, f"{case}, lists {lists}")
+
+
+def test_advect_pass_bit_exact(oracle):
+    """Fluid::runAdvectProg (advect.comp:20-59) on buffer 1, after a few plain steps and with
+    velocities that push particles through every wall."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    rng = np.random.default_rng(3)
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 5)
+    with Fluid(p, parts) as f:
+        f.step(dt, 5)
+        state = f.particles(1)
+        _check_records(state, st.buf1, "before advect")
+        state["velocity"] = rng.uniform(-60.0, 60.0, (state.shape[0], 3)).astype(np.float32)
+        f.upload(state)
+        ref = oracle.OracleState(p, state)
+        for _ in range(3):
+            f.runAdvectProg(2e-3)
+            ref.advect(2e-3)
+        _check_records(f.particles(1), ref.buf1, "advect")

@@ -415,3 +415,31 @@ def test_cutoff_by_squared_distance_is_exact():
         bits = cut2.view(np.uint32).astype(np.int64) + np.arange(-4096, 4097)
         d2 = bits.astype(np.uint32).view(np.float32)
         assert np.array_equal(np.sqrt(d2) >= h, d2 >= cut2)
+
+
+def test_advect_single_step_outcomes(oracle):
+    """advect.comp:20-59: x + v dt, then the clamp at the shader's own border 0.01 and damping 0.3
+    (the parameter block's border and damping are set to other values and must not be used)."""
+    p = _params(num_particles=4, border=0.002, wall_damping=0.5)
+    parts = np.zeros(4, L.PARTICLE_DTYPE)
+    parts["position"] = [[0.5, 0.5, 0.5], [0.02, 0.5, 0.5], [0.5, 0.995, 0.5], [0.5, 0.5, 0.0105]]
+    parts["velocity"] = [[0.25, -0.5, 1.0], [-1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, -0.01]]
+    parts["density"] = [1.0, 2.0, 3.0, 4.0]
+    parts["pressure"] = [5.0, 6.0, 7.0, 8.0]
+    st = oracle.OracleState(p, parts)
+    st.advect(0.1)
+    x, v = st.buf1["position"], st.buf1["velocity"]
+    f = np.float32
+    # interior: exact sums at these values (0.5 + 0.025 etc. rounded once each)
+    assert x[0].tolist() == [f(f(0.5) + f(f(0.25) * f(0.1))), f(f(0.5) + f(f(-0.5) * f(0.1))),
+                             f(f(0.5) + f(f(1.0) * f(0.1)))]
+    assert v[0].tolist() == [f(0.25), f(-0.5), f(1.0)]
+    # low wall: x = 0.02 - 0.1 < 0.01 -> 0.01, v = -1 * -0.3
+    assert x[1][0] == f(0.01) and v[1][0] == f(f(-1.0) * f(-0.3))
+    # high wall: 0.995 + 0.1 > size - 0.01 -> size - 0.01, v = 1 * -0.3
+    assert x[2][1] == f(f(p.size) - f(0.01)) and v[2][1] == f(-0.3)
+    # 0.0105 - 0.001 = 0.0095 < 0.01: clamped; the parameter block's border 0.002 would not clamp
+    assert x[3][2] == f(0.01) and v[3][2] == f(f(-0.01) * f(-0.3))
+    # density and pressure pass through
+    assert st.buf1["density"].tolist() == [1.0, 2.0, 3.0, 4.0]
+    assert st.buf1["pressure"].tolist() == [5.0, 6.0, 7.0, 8.0]

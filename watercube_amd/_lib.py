@@ -127,6 +127,8 @@ SIGNATURES = {
     "wc_sort_run": (C.c_int, [_vp]),
     "wc_density": (C.c_int, [_vp]),
     "wc_update": (C.c_int, [_vp, C.c_float]),
+    "wc_advect": (C.c_int, [_vp, C.c_float]),
+    "wc_gravity_direction": (C.c_int, [_P(C.c_float), _P(C.c_float)]),
     "wc_step": (C.c_int, [_vp, C.c_float, C.c_uint32]),
     "wc_synchronize": (C.c_int, [_vp]),
     "wc_create_batch": (C.c_int, [_P(wc_params), C.c_uint32, _P(_vp)]),

@@ -584,6 +584,18 @@ WC_API int wc_destroy(wc_ctx* c) {
 
 WC_API int wc_num_scenes(const wc_ctx* c) { return c ? (int)c->num_scenes : fail(WC_ERR_INVALID_ARG, "ctx is NULL"); }
 
+WC_API int wc_gravity_direction(const float q[4], float out[3]) {
+    if (!q || !out) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    const float w = q[0], x = q[1], y = q[2], z = q[3];
+    // column 1 of glm::toMat3(q): (2(xy + wz), 1 - 2(xx + zz), 2(yz - wx)); transpose(M) * (0, -1, 0)
+    // is minus that column
+    const float xx = x * x, zz = z * z, xy = x * y, wz = w * z, yz = y * z, wx = w * x;
+    out[0] = -(2.0f * (xy + wz));
+    out[1] = -(1.0f - 2.0f * (xx + zz));
+    out[2] = -(2.0f * (yz - wx));
+    return WC_OK;
+}
+
 WC_API int wc_set_params(wc_ctx* c, const wc_params* p) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     int rc = check_params(p);
@@ -663,6 +675,14 @@ WC_API int wc_update(wc_ctx* c, float dt) {
     rc = set_device(c);
     if (rc) return rc;
     WC_HIP(wc::launch_update(c->b, c->update_variant, c->neighbour_list, c->stream));
+    return WC_OK;
+}
+
+WC_API int wc_advect(wc_ctx* c, float dt) {
+    if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
+    int rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(wc::launch_advect(c->b, dt, c->stream));
     return WC_OK;
 }
 

@@ -135,6 +135,7 @@ hipError_t launch_gather(const DeviceBuffers& b, hipStream_t stream);
 // With `list`, the density pass also writes the neighbour lists and the force pass reads them.
 hipError_t launch_density(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
 hipError_t launch_update(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
+hipError_t launch_advect(const DeviceBuffers& b, float dt, hipStream_t stream);
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n,
                                 hipStream_t stream);
 hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,

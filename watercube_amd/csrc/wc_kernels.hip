@@ -861,6 +861,41 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
 
 
 // ------------------------------------------------------------------------------------------
+// advect.comp:20-59 (the reference compiles it but Fluid::update never dispatches it): a
+// position-only step on buffer 1, x += v dt, then the cube clamp with the shader's own border
+// 0.01 and damping 0.3 (advect.comp:33-34), in place.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kHashBlock) void k_advect(Scenes S, float4* __restrict__ posrho,
+                                                       float4* __restrict__ velp, float dt) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+    const SceneDesc& d = S.desc[s];
+    const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
+    if (lt >= d.c.n) return;
+    const uint32_t i = d.p_off + lt;
+    float4 x = posrho[i], v = velp[i];
+    const float damping = 0.3f, border = 0.01f;
+    const float hi = d.c.size - border;
+    x.x = x.x + v.x * dt;
+    x.y = x.y + v.y * dt;
+    x.z = x.z + v.z * dt;
+    auto clamp_axis = [&](float& xa, float& va) {
+        if (xa < border) {
+            va = va * -damping;
+            xa = border;
+        } else if (xa > hi) {
+            va = va * -damping;
+            xa = hi;
+        }
+    };
+    clamp_axis(x.x, v.x);
+    clamp_axis(x.y, v.y);
+    clamp_axis(x.z, v.z);
+    posrho[i] = x;
+    velp[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------
 // Learned closure features (single-scene contexts).  For every requested slot of the sorted
 // buffer: its 32 nearest neighbours inside the kernel radius (the density pass's acceptance
 // test; nearest by squared distance, among equal distances the earlier in NEIGHBORHOOD
@@ -1179,6 +1214,12 @@ hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* 
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
                            reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_advect(const DeviceBuffers& b, float dt, hipStream_t s) {
+    if (b.sc.blocks256 == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_advect, dim3(b.sc.blocks256), dim3(kHashBlock), 0, s, b.sc, b.posrho[0], b.velp[0], dt);
     return hipGetLastError();
 }
 

@@ -13,6 +13,15 @@ from . import _lib as L
 from .scenes import reference_scene
 
 
+def gravity_direction(q_wxyz) -> np.ndarray:
+    """Fluid::updateGravity's direction (src/core/Fluid.cpp:259-263) for the fluid's orientation
+    quaternion q = (w, x, y, z): transpose(toMat3(q)) * (0, -1, 0) (wc_gravity_direction)."""
+    q = (C.c_float * 4)(*[float(v) for v in q_wxyz])
+    out = (C.c_float * 3)()
+    L.check(L.lib().wc_gravity_direction(q, out), "wc_gravity_direction")
+    return np.array(out[:], np.float32)
+
+
 class Sort:
     """``core::Sort`` (src/core/Sort.h:20-71): the grid build of one simulation context.
 
@@ -98,6 +107,18 @@ class Fluid:
 
     def runUpdateProg(self, dt: float) -> None:  # noqa: N802  (src/core/Fluid.cpp:294-321)
         L.check(L.lib().wc_update(self.ctx, C.c_float(dt)), "wc_update")
+
+    def runAdvectProg(self, dt: float) -> None:  # noqa: N802  (src/core/Fluid.cpp:326-336, on buffer 1)
+        L.check(L.lib().wc_advect(self.ctx, C.c_float(dt)), "wc_advect")
+
+    def updateGravity(self, q_wxyz, strength: float) -> None:  # noqa: N802  (src/core/Fluid.cpp:259-263)
+        """Rotating gravity: direction from the fluid's orientation q = (w, x, y, z), times
+        `strength` (src/core/Fluid.cpp:310), pushed to the context."""
+        d = gravity_direction(q_wxyz)
+        for a in range(3):
+            self.params.gravity[a] = float(np.float32(d[a]) * np.float32(strength))
+        if self.ctx:
+            L.check(L.lib().wc_set_params(self.ctx, C.byref(self.params)), "wc_set_params")
 
     # -- extras -----------------------------------------------------------------------------
     def step(self, dt: float, nsteps: int = 1) -> None:
