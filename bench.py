@@ -13,6 +13,8 @@ Besides the throughput the line carries
                committed rocprofv3 FETCH_SIZE / WRITE_SIZE profile of that kernel measured, and
                `valu` its VALU issue rate (SQ_INSTS_VALU of the committed profile over the live
                duration) against one wave64 instruction per CU per cycle;
+  pass_ms      the average duration of each of the six kernels of the step, and pass_gbs the
+               same passes' algorithmic and measured HBM bytes per launch over it (GB/s);
   batched      256 independent scenes of 4,096..262,144 particles stepped together;
   scale        the 17,777,216-particle scene;
   learned      plain and learned steps of K*dt against plain steps of dt (density errors,
@@ -330,6 +332,11 @@ def main() -> int:
                               "neighbour_list": tuning["neighbour_list"]}},
         "simulated_s_per_s": dt * args.steps / elapsed,
         "pass_ms": per_launch_ms,
+        # per pass: algorithmic bytes and rocprofv3-measured HBM bytes (committed summary) over
+        # the live per-launch duration, in GB/s
+        "pass_gbs": {k: {"algorithmic": algo[k] / (v * 1e-3) / 1e9,
+                         "measured": None if read_traffic(k) is None else read_traffic(k) / (v * 1e-3) / 1e9}
+                     for k, v in per_launch_ms.items() if v > 0},
         "roofline": {
             "kernel": dom,
             "bound": "hbm",
