@@ -274,8 +274,8 @@ def test_cpp_harness_matches_oracle(tmp_path, oracle):
 def test_block_sphere_17m_bit_exact_and_invariants(oracle):
     """The 17,777,216-particle scene (256^3 block plus a falling 1M ball; R = 250, 15.6 M cells):
     the grid build at scale (look-back scan over 3,815 tiles) and one whole step bit-exact
-    against the CPU restatement; then, after 20 more GPU steps, the cell invariants of the next
-    sort and one further step bit-exact again."""
+    against the CPU restatement; then, after the rest of the workload's 200 steps on the GPU, the
+    cell invariants of the next sort and one further step bit-exact again."""
     p, parts, dt, _ = scenes.workload("block_sphere_17m")
     n = int(p.num_particles)
     _log(f"block_sphere_17m n={n} R={p.grid_res}: sort + one step")
@@ -291,8 +291,8 @@ def test_block_sphere_17m_bit_exact_and_invariants(oracle):
         f.runUpdateProg(dt)
         st.update(dt)
         _check_records(f.particles(1), st.buf1, "17m update pass")
-        _log("block_sphere_17m: 20 more steps")
-        f.step(dt, 20)
+        _log("block_sphere_17m: 199 more steps")
+        f.step(dt, 199)
         f.sort_.run()
         f.synchronize()
         counts = f.sort_.getCountBuffer()
@@ -313,7 +313,7 @@ def test_block_sphere_17m_bit_exact_and_invariants(oracle):
     st.step(dt, 1)
     with Fluid(p, buf1) as g:
         g.step(dt, 1)
-        _check_records(g.particles(1), st.buf1, "17m step from the step-21 state")
+        _check_records(g.particles(1), st.buf1, "17m step from the step-200 state")
 
 
 def test_resume_from_snapshot_equals_uninterrupted_run(tmp_path, oracle):
