@@ -57,3 +57,21 @@ def test_batch_tuning_keeps_results(oracle):
             st = oracle.OracleState(p, parts)
             st.step(dt, 5)
             assert b.scene_particles(s).tobytes() == st.buf1.tobytes()
+
+
+def test_batch_with_one_out_of_range_scene(oracle):
+    """The force pass's division mode is one flag for the whole batch: a single velocity of 1e-30
+    in one scene switches every scene's step to IEEE division.  Each scene must still match the
+    CPU restatement bit for bit, the altered one and the others alike."""
+    scenes = _small_batch()
+    p, parts, dt, kind = scenes[2]
+    parts = parts.copy()
+    parts["velocity"][5, 0] = np.float32(1e-30)
+    scenes[2] = (p, parts, dt, kind)
+    with batch.SceneBatch(scenes) as b:
+        b.step(4)
+        b.synchronize()
+        for s, (p, parts, dt, kind) in enumerate(scenes):
+            st = oracle.OracleState(p, parts)
+            st.step(dt, 4)
+            assert b.scene_particles(s).tobytes() == st.buf1.tobytes(), f"scene {s} ({kind}) differs"
