@@ -462,6 +462,7 @@ def main(argv=None) -> int:
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
     e.add_argument("--horizon", type=int, default=100)
+    e.add_argument("--ks", default="2,4", help="step multipliers K to compare, comma-separated")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -484,7 +485,8 @@ def main(argv=None) -> int:
         log(json.dumps(ev))
     else:
         net = ClosureNet(max_batch=1024).load(args.weights)
-        log(json.dumps(evaluate(net, horizon=args.horizon, log=log)))
+        ks = tuple(int(k) for k in args.ks.split(","))
+        log(json.dumps(evaluate(net, horizon=args.horizon, ks=ks, log=log)))
     return 0
 
 
