@@ -37,6 +37,20 @@ int fail(int code, const char* fmt, ...) {
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// A start/stop pair of timing events, destroyed on every return path.
+struct EventPair {
+    hipEvent_t start = nullptr, stop = nullptr;
+    hipError_t create() {
+        hipError_t e = hipEventCreate(&start);
+        if (e == hipSuccess) e = hipEventCreate(&stop);
+        return e;
+    }
+    ~EventPair() {
+        if (start) (void)hipEventDestroy(start);
+        if (stop) (void)hipEventDestroy(stop);
+    }
+};
+
 }  // namespace
 
 namespace wc {
@@ -882,9 +896,8 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
             (void)hipFree(save_pos);
             return fail(WC_ERR_NO_MEMORY, "tuning copy of %zu particles", n);
         }
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        hipError_t err = hipEventCreate(&e0);
-        if (err == hipSuccess) err = hipEventCreate(&e1);
+        EventPair ev;
+        hipError_t err = ev.create();
         if (err == hipSuccess)
             err = hipMemcpyAsync(save_pos, c->b.posrho[0], n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
         if (err == hipSuccess)
@@ -901,12 +914,12 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
                                          : wc::launch_update(c->b, v, mode == 1, c->stream);
                     };
                     err = launch();  // warm-up
-                    if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
+                    if (err == hipSuccess) err = hipEventRecord(ev.start, c->stream);
                     for (uint32_t r = 0; r < reps && err == hipSuccess; ++r) err = launch();
-                    if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
-                    if (err == hipSuccess) err = hipEventSynchronize(e1);
+                    if (err == hipSuccess) err = hipEventRecord(ev.stop, c->stream);
+                    if (err == hipSuccess) err = hipEventSynchronize(ev.stop);
                     float ms = 0.0f;
-                    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                    if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev.start, ev.stop);
                     t[mode][pass][v] = (double)ms / reps;
                 }
             }
@@ -916,8 +929,6 @@ WC_API int wc_tune(wc_ctx* c, float dt, uint32_t reps, wc_tuning* chosen, double
         if (err == hipSuccess)
             err = hipMemcpyAsync(c->b.velp[0], save_vel, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
         if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
         (void)hipFree(save_pos);
         (void)hipFree(save_vel);
         if (err != hipSuccess) return fail(WC_ERR_HIP, "tuning failed: %s", hipGetErrorString(err));
@@ -1184,19 +1195,16 @@ WC_API int wc_mlp_train_step(wc_mlp* h, const void* x_dev, const void* t_dev, ui
 WC_API int wc_mlp_profile_predict(wc_mlp* h, const void* x_dev, void* y_dev, uint32_t n, uint32_t reps, double* ms) {
     if (!h || !ms || ((!x_dev || !y_dev) && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
     if (reps == 0) reps = 1;
-    hipEvent_t e0, e1;
-    WC_HIP(hipEventCreate(&e0));
-    WC_HIP(hipEventCreate(&e1));
-    hipError_t err = wc::mlp_infer(h->m, (const uint16_t*)x_dev, (float*)y_dev, n, h->m.stream);
-    if (err == hipSuccess) err = hipEventRecord(e0, h->m.stream);
+    EventPair ev;
+    hipError_t err = ev.create();
+    if (err == hipSuccess) err = wc::mlp_infer(h->m, (const uint16_t*)x_dev, (float*)y_dev, n, h->m.stream);
+    if (err == hipSuccess) err = hipEventRecord(ev.start, h->m.stream);
     for (uint32_t r = 0; r < reps && err == hipSuccess; ++r)
         err = wc::mlp_infer(h->m, (const uint16_t*)x_dev, (float*)y_dev, n, h->m.stream);
-    if (err == hipSuccess) err = hipEventRecord(e1, h->m.stream);
-    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    if (err == hipSuccess) err = hipEventRecord(ev.stop, h->m.stream);
+    if (err == hipSuccess) err = hipEventSynchronize(ev.stop);
     float t = 0.0f;
-    if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, ev.start, ev.stop);
     if (err != hipSuccess) return fail(WC_ERR_HIP, "timed inference: %s", hipGetErrorString(err));
     *ms = (double)t / reps;
     return WC_OK;
@@ -1295,10 +1303,9 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
     const float g = 9.81f;
     if (c->b.ids[0] && !c->cl_backup_ids && hipMalloc(&c->cl_backup_ids, (size_t)(n ? n : 1) * 4) != hipSuccess)
         return fail(WC_ERR_NO_MEMORY, "identity backup");
-    hipEvent_t e0, e1;
-    WC_HIP(hipEventCreate(&e0));
-    WC_HIP(hipEventCreate(&e1));
-    WC_HIP(hipEventRecord(e0, c->stream));
+    EventPair ev;
+    WC_HIP(ev.create());
+    WC_HIP(hipEventRecord(ev.start, c->stream));
     float prev_max = 0.0f;
     // Judge the learned step that produced buffer 1: sort + density of the new state, compare
     // the density maximum with the one before the step; past the tolerance, replace the step by
@@ -1369,15 +1376,13 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         rc = judge(&now);
     }
     if (!rc) {
-        hipError_t e = hipEventRecord(e1, c->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        hipError_t e = hipEventRecord(ev.stop, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ev.stop);
         float ms = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev.start, ev.stop);
         if (e != hipSuccess) rc = fail(WC_ERR_HIP, "closure timing: %s", hipGetErrorString(e));
         stats->wall_ms = ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return rc;
 }
 
