@@ -429,14 +429,17 @@ def test_force_pass_division_modes(lists, oracle):
             _check_records(f.particles(1), st.buf1, f"{what}, lists {lists}")
 
 
-def test_force_pass_right_after_a_sort(oracle):
+@pytest.mark.parametrize("lists", [0, 1])
+def test_force_pass_right_after_a_sort(lists, oracle):
     """A force pass straight after a sort, with no density pass between them (the densities are
     then the ones the records carried in): it must not rely on the last density pass's operand
-    check, which describes an older sorted buffer, and so divides as IEEE does."""
+    check or neighbour lists, which describe an older sorted buffer, and so divides as IEEE does
+    and searches for its neighbours."""
     p, parts, dt, _ = scenes.workload("dam_break_16k")
     st = oracle.OracleState(p, parts)
     st.step(dt, 3)
     with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, lists)
         f.step(dt, 3)  # leaves a density pass's state behind
         # move the particles, then sort and update only
         moved = f.particles(1)

@@ -775,7 +775,9 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
         };
         bool searched = true;
         if constexpr (LIST) {
-            const uint32_t cnt = ncount[t];
+            // lists are written by the density pass; after a sort with none since, they describe
+            // the previous order and the particle searches instead
+            const uint32_t cnt = (div_flags[0] & kDivPending) ? kListOverflow : ncount[t];
             if (cnt != kListOverflow) {
                 searched = false;
                 // the same loop as the drain, over the stored list (entry k at nlist[k * total + t]):
