@@ -186,6 +186,47 @@ def test_particles_outside_the_cube(oracle):
     _trajectory_case(p, _records(np.float32(pos), np.float32([[3, -4, 5]] * len(pos))), 2e-4, 5, oracle, "outside")
 
 
+def _check_records_nan_aware(gpu, ref, what):
+    """NaN where the oracle has NaN (payload and sign may differ: the two sides' NaN-producing
+    instructions pick different canonical NaNs), the same bits everywhere else."""
+    for f in ("position", "density", "velocity", "pressure"):
+        x, y = gpu[f], ref[f]
+        nx, ny = np.isnan(x), np.isnan(y)
+        assert np.array_equal(nx, ny), f"{what}: {f} NaN at {np.count_nonzero(nx)} vs {np.count_nonzero(ny)} places"
+        assert np.array_equal(x[~nx].view(np.uint32), y[~ny].view(np.uint32)), f"{what}: {f} differs"
+
+
+@pytest.mark.parametrize("lists", [0, 1])
+def test_nan_and_infinite_inputs(lists, oracle):
+    """A NaN position poisons the density and pressure of every particle within reach (a NaN
+    distance passes the cutoff test, as `dist >= h` is false, density.comp:117); clamping the speed
+    with min/max then drops the NaN from the velocity (update.comp:199). Infinite positions hash
+    to the last cell and are pulled back by the wall clamp. Every step must agree with the oracle,
+    and cell counts, offsets and order exactly."""
+    rng = np.random.default_rng(5)
+    n0 = 300
+    pos = rng.uniform(0.01, 0.07, (n0, 3)).astype(np.float32)
+    extra_pos = np.float32([[np.nan, 0.03, 0.03], [np.inf, 0.5, 0.5], [0.04, -np.inf, 0.04], [0.5, 0.5, 0.5],
+                            [0.05, 0.05, 0.05]])
+    extra_vel = np.float32([[0, 0, 0], [0, 0, 0], [0, 0, 0], [np.nan, 1, 1], [np.inf, -np.inf, 0]])
+    parts = _records(np.concatenate([pos, extra_pos]))
+    parts["velocity"][n0:] = extra_vel
+    p = _params_for(parts.size)
+    _log(f"case nan/inf inputs lists={lists}: n={parts.size}")
+    st = oracle.OracleState(p, parts)
+    with Fluid(p, parts) as f:
+        f.set_tuning(0, 0, lists)
+        for s in range(6):
+            st.step(2e-4, 1)
+            f.step(2e-4, 1)
+            f.synchronize()
+            _check_records_nan_aware(f.particles(1), st.buf1, f"step {s + 1}")
+            assert np.array_equal(f.sort_.getCountBuffer(), st.counts)
+            assert np.array_equal(f.sort_.getOffsetBuffer(), st.offsets)
+            assert np.array_equal(f.sort_.getSortedBuffer(), st.sorted[: parts.size])
+        assert np.isnan(st.buf1["density"]).sum() > 200  # the NaN did spread
+
+
 def test_dense_cluster_uses_fallback_path(oracle):
     """400 particles packed into a quarter-cell: every particle has ~399 neighbours, more than the
     neighbour list holds, so the force pass walks the cells again; results must not change."""
