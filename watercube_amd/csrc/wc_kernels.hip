@@ -379,12 +379,13 @@ __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t*
 // Each lane owns one particle.  The 27 neighbour cells are visited in NEIGHBORHOOD order with a
 // wave-uniform cell loop; inside a cell each lane walks its own slot range CHUNK slots at a time
 // (CHUNK loads in flight).  Candidate tests are branch-free: every candidate is written to the
-// lane's next entry of an LDS list and the entry count advances by the test's outcome.  The list
-// is drained when the next chunk could overflow it, at the end, and (PLANE_DRAIN) after every dx
-// plane: the per-neighbour arithmetic then runs over compacted entries with all lanes busy
-// instead of under a mask where ~15% of the lanes accept a given candidate.  Entries are drained
-// in enumeration order, so every sum is formed exactly as the reference forms it, and the
-// result does not depend on BLOCK, CAP, CHUNK or PLANE_DRAIN (the tuning knobs of the passes).
+// lane's next entry of an LDS list and the entry count advances by the test's outcome.  The lists
+// are drained at the end, (PLANE_DRAIN) after every dx plane, and whenever one lane's list could
+// overflow on its next chunk; every lane drains then, not only the full ones.  The per-neighbour
+// arithmetic thus runs over compacted entries with all lanes busy instead of under a mask where
+// ~15% of the lanes accept a given candidate.  Entries are drained in enumeration order, so
+// every sum is formed exactly as the reference forms it, and the result does not depend on
+// BLOCK, CAP, CHUNK or PLANE_DRAIN (the tuning knobs of the passes) or on when the lists drain.
 // ------------------------------------------------------------------------------------------
 // Pointers into LDS (32-bit addresses, so list arithmetic stays 32-bit).
 template <typename T>
@@ -405,8 +406,12 @@ __device__ __forceinline__ void search_neighbours(uint32_t key, uint32_t t, uint
         next = list;
     };
     auto visit = [&](uint32_t start, uint32_t end, auto own) {
-        for (uint32_t u = start; u < end; u += CHUNK) {
-            if (next > full) drain();
+        // the chunks of a cell run in step across the wave: every lane visiting the cell takes part
+        // in each iteration (with its own slots, or none once its range is done), so when one list
+        // fills, every lane drains with it instead of waiting on a drain of a few lanes
+        for (uint32_t u = start; __any(u < end); u += CHUNK) {
+            if (__any(next > full)) drain();
+            if (u >= end) continue;
             const uint32_t left = end - u;  // slots of the cell from u on (>= 1 here): u + j is one iff j < left
             // slots past the cell are loaded (the buffers are padded) and rejected below
             const float4* q = pos + u;
