@@ -1307,9 +1307,9 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
     WC_HIP(ev.create());
     WC_HIP(hipEventRecord(ev.start, c->stream));
     float prev_max = 0.0f;
-    // Judge the learned step that produced buffer 1: sort + density of the new state, compare
-    // the density maximum with the one before the step; past the tolerance, replace the step by
-    // K plain steps from the saved state.
+    // Judge a learned step from the density pass of its new state: compare the density maximum
+    // with the one before the step; past the tolerance, replace the step by K plain steps from
+    // the saved state.
     auto sort_density = [&]() -> int {
         int r = enqueue_sort(c);
         if (r) return r;
@@ -1317,9 +1317,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         return WC_OK;
     };
     auto judge = [&](float* now) -> int {
-        int r = sort_density();
-        if (r) return r;
-        r = read_rho_max(c, now);
+        int r = read_rho_max(c, now);
         if (r) return r;
         const float jump = (*now - prev_max) / rho0;
         if ((double)jump > stats->max_density_jump) stats->max_density_jump = jump;
@@ -1341,39 +1339,58 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         }
         return WC_OK;
     };
+    // One macro step without its judge's verdict: save the sorted state, form the features, run
+    // the network, take the large step, then sort and run the density pass of the new state (the
+    // judge's input, and the next step's).  Captured once per call and replayed, so the GPU does
+    // not wait on a dozen launches after each verdict.
+    auto enqueue_macro = [&]() -> int {
+        WC_HIP(hipMemcpyAsync(c->cl_backup[0], c->b.posrho[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
+                              c->stream));
+        WC_HIP(hipMemcpyAsync(c->cl_backup[1], c->b.velp[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
+                              c->stream));
+        if (c->b.ids[0])
+            WC_HIP(hipMemcpyAsync(c->cl_backup_ids, c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (net) {
+            WC_HIP(wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->neighbour_list,
+                                               c->stream));
+            WC_HIP(wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream));
+        }
+        WC_HIP(wc::launch_closure_update(c->b, dt_large, net ? c->cl_y : nullptr, g, nullptr, c->cl_smooth,
+                                         c->cl_substeps ? (float)K : 0.0f, c->neighbour_list, c->stream));
+        return sort_density();
+    };
+    struct MacroGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ~MacroGraph() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+    } mg;
+    // the network's own stream may still be writing its bf16 weights
+    if (net) WC_HIP(hipStreamSynchronize(net->m.stream));
     rc = sort_density();
     if (!rc) rc = read_rho_max(c, &prev_max);
-    for (uint32_t m = 0; m < nmacro && !rc; ++m) {
-        if (m > 0) {
-            float now = 0.0f;
-            rc = judge(&now);
-            if (rc) break;
-            prev_max = now;
-        }
-        // save the state at the start of this step (sorted, with densities) for a fallback
-        hipError_t e = hipMemcpyAsync(c->cl_backup[0], c->b.posrho[1], (size_t)n * sizeof(float4),
-                                      hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(c->cl_backup[1], c->b.velp[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
-                               c->stream);
-        if (e == hipSuccess && c->b.ids[0])
-            e = hipMemcpyAsync(c->cl_backup_ids, c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream);
-        if (net) {
-            if (e == hipSuccess)
-                e = wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->neighbour_list,
-                                                c->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(net->m.stream);
-            if (e == hipSuccess) e = wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream);
-        }
-        if (e == hipSuccess)
-            e = wc::launch_closure_update(c->b, dt_large, net ? c->cl_y : nullptr, g, nullptr, c->cl_smooth,
-                                          c->cl_substeps ? (float)K : 0.0f, c->neighbour_list, c->stream);
-        if (e != hipSuccess) rc = fail(WC_ERR_HIP, "learned step: %s", hipGetErrorString(e));
-        stats->macro_steps += 1;
-    }
     if (!rc && nmacro > 0) {
+        WC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_macro();
+        const hipError_t e = hipStreamEndCapture(c->stream, &mg.graph);
+        if (!rc && e != hipSuccess) rc = fail(WC_ERR_HIP, "learned step capture: %s", hipGetErrorString(e));
+        if (!rc) {
+            const hipError_t ei = hipGraphInstantiate(&mg.exec, mg.graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess) rc = fail(WC_ERR_HIP, "learned step graph: %s", hipGetErrorString(ei));
+        }
+    }
+    for (uint32_t m = 0; m < nmacro && !rc; ++m) {
+        const hipError_t e = hipGraphLaunch(mg.exec, c->stream);
+        if (e != hipSuccess) {
+            rc = fail(WC_ERR_HIP, "learned step: %s", hipGetErrorString(e));
+            break;
+        }
+        stats->macro_steps += 1;
         float now = 0.0f;
         rc = judge(&now);
+        if (!rc) prev_max = now;
     }
     if (!rc) {
         hipError_t e = hipEventRecord(ev.stop, c->stream);

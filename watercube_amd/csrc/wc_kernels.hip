@@ -947,6 +947,96 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
+// One feature row: the kept neighbours' relative positions (/ h) and velocities (* inv_v), in
+// bf16, then the scaled density deviation, dt multiplier and filled fraction.  slot_of(k) gives
+// the k-th kept neighbour's slot, k < cnt.
+// Software-pipelined: the gathers of group g + 1 are issued before group g is formed, and the
+// scheduling barrier keeps the compiler from interleaving them with waits.  Positions k >= cnt
+// load the particle's own record t and write zeros.
+template <typename SlotOf>
+__device__ __forceinline__ void write_feature_row(uint4* __restrict__ row, const StepConsts& c, uint32_t t, float4 p,
+                                                  float4 pv, int cnt, SlotOf slot_of,
+                                                  const float4* __restrict__ posrho, const float4* __restrict__ velp,
+                                                  float dt_mult, float inv_v) {
+    const float inv_h = 1.0f / c.h;
+    float4 q[4], qv[4];
+    auto load_group = [&](int g) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = g * 4 + j;
+            const uint32_t u = k < cnt ? slot_of(k) : t;
+            q[j] = posrho[u];
+            qv[j] = velp[u];
+        }
+    };
+    load_group(0);
+    // eight groups of four neighbours: 24 values = three 16-byte stores
+#pragma unroll
+    for (int g = 0; g < kClosureNeighbours / 4; ++g) {
+        float4 cq[4], cqv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cq[j] = q[j];
+            cqv[j] = qv[j];
+        }
+        if (g + 1 < kClosureNeighbours / 4) load_group(g + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        float v[24];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            // zeros past cnt by masking the bits; the mask is opaque to the optimizer, which would
+            // otherwise turn a select into a branch around the loads
+            uint32_t m = g * 4 + j < cnt ? ~0u : 0u;
+            asm("" : "+v"(m));
+            auto keep = [m](float x) { return __uint_as_float(__float_as_uint(x) & m); };
+            v[6 * j + 0] = keep((cq[j].x - p.x) * inv_h);
+            v[6 * j + 1] = keep((cq[j].y - p.y) * inv_h);
+            v[6 * j + 2] = keep((cq[j].z - p.z) * inv_h);
+            v[6 * j + 3] = keep((cqv[j].x - pv.x) * inv_v);
+            v[6 * j + 4] = keep((cqv[j].y - pv.y) * inv_v);
+            v[6 * j + 5] = keep((cqv[j].z - pv.z) * inv_v);
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+            row[g * 3 + e] = make_uint4(pack_bf16(v[8 * e + 0], v[8 * e + 1]), pack_bf16(v[8 * e + 2], v[8 * e + 3]),
+                                        pack_bf16(v[8 * e + 4], v[8 * e + 5]), pack_bf16(v[8 * e + 6], v[8 * e + 7]));
+    }
+    const float rho_rel = (p.w / c.rho0 - 1.0f) * 10.0f;
+    row[24] = make_uint4(pack_bf16(rho_rel, dt_mult * 0.25f), pack_bf16((float)cnt * (1.0f / kClosureNeighbours), 0.0f),
+                         0u, 0u);
+    row[25] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Rows whose particle has at most kClosureNeighbours listed neighbours keep all of them, in list
+// order: written straight from the density pass's lists, with no distances and no LDS.  Other
+// rows are left to k_closure_features.
+__global__ __launch_bounds__(256) void k_closure_features_short(Scenes S, const float4* __restrict__ posrho,
+                                                                const float4* __restrict__ velp,
+                                                                const uint32_t* __restrict__ rows, uint32_t nrows,
+                                                                float dt_mult, float inv_v, uint16_t* __restrict__ X,
+                                                                const uint32_t* __restrict__ nlist,
+                                                                const uint32_t* __restrict__ ncount) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nrows) return;
+    const SceneDesc& d = S.desc[0];
+    const StepConsts& c = d.c;
+    const uint32_t t = rows ? rows[r] : d.p_off + r;
+    uint4* row = reinterpret_cast<uint4*>(X + (size_t)r * kClosureFeatures);
+    if (t - d.p_off >= c.n) {
+        for (int q = 0; q < kClosureFeatures / 8; ++q) row[q] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const uint32_t listed = ncount[t];
+    if (listed > (uint32_t)kClosureNeighbours) return;  // more than are kept, or overflowed
+    // all listed slots first (coalesced across the wave, in flight together)
+    const size_t stride = S.total_particles;
+    uint32_t u[kClosureNeighbours];
+#pragma unroll
+    for (int k = 0; k < kClosureNeighbours; ++k) u[k] = (uint32_t)k < listed ? nlist[(size_t)k * stride + t] : t;
+    write_feature_row(row, c, t, posrho[t], velp[t], (int)listed, [&](int k) { return u[k]; }, posrho, velp, dt_mult,
+                      inv_v);
+}
+
 __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const float4* __restrict__ posrho,
                                                                  const float4* __restrict__ velp,
                                                                  const uint32_t* __restrict__ skey,
@@ -954,7 +1044,8 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
                                                                  const uint32_t* __restrict__ rows, uint32_t nrows,
                                                                  float dt_mult, float inv_v, uint16_t* __restrict__ X,
                                                                  const uint32_t* __restrict__ nlist,
-                                                                 const uint32_t* __restrict__ ncount) {
+                                                                 const uint32_t* __restrict__ ncount,
+                                                                 bool skip_short) {
     static_assert(kFeatChunk <= (int)kRecordPad + 1, "chunked loads may run kFeatChunk - 1 records past a cell");
     static_assert(kFeatCap >= kClosureNeighbours + kFeatChunk, "a chunk must fit after a selection");
     __shared__ uint32_t s_key[kFeatCap * kFeatBlock];
@@ -969,6 +1060,8 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
         for (int q = 0; q < kClosureFeatures / 8; ++q) row[q] = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
+    // rows of short lists are written by k_closure_features_short
+    if (skip_short && ncount[t] <= (uint32_t)kClosureNeighbours) return;
     const float4 p = posrho[t];
     const float4 pv = velp[t];
     const lds_ptr<uint32_t> key = (lds_ptr<uint32_t>)s_key + threadIdx.x;
@@ -1022,37 +1115,8 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
     }
     if (cnt > kClosureNeighbours) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
 
-    const float inv_h = 1.0f / c.h;
-    // eight groups of four neighbours: 24 values = three 16-byte stores
-    for (int g = 0; g < kClosureNeighbours / 4; ++g) {
-        float v[24];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = g * 4 + j;
-            if (k < cnt) {
-                const uint32_t u = slot[k * kFeatBlock];
-                const float4 q = posrho[u];
-                const float4 qv = velp[u];
-                v[6 * j + 0] = (q.x - p.x) * inv_h;
-                v[6 * j + 1] = (q.y - p.y) * inv_h;
-                v[6 * j + 2] = (q.z - p.z) * inv_h;
-                v[6 * j + 3] = (qv.x - pv.x) * inv_v;
-                v[6 * j + 4] = (qv.y - pv.y) * inv_v;
-                v[6 * j + 5] = (qv.z - pv.z) * inv_v;
-            } else {
-#pragma unroll
-                for (int e = 0; e < 6; ++e) v[6 * j + e] = 0.0f;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-            row[g * 3 + q] = make_uint4(pack_bf16(v[8 * q + 0], v[8 * q + 1]), pack_bf16(v[8 * q + 2], v[8 * q + 3]),
-                                        pack_bf16(v[8 * q + 4], v[8 * q + 5]), pack_bf16(v[8 * q + 6], v[8 * q + 7]));
-    }
-    const float rho_rel = (p.w / c.rho0 - 1.0f) * 10.0f;
-    row[24] = make_uint4(pack_bf16(rho_rel, dt_mult * 0.25f), pack_bf16((float)cnt * (1.0f / kClosureNeighbours), 0.0f),
-                         0u, 0u);
-    row[25] = make_uint4(0u, 0u, 0u, 0u);
+    write_feature_row(row, c, t, p, pv, cnt, [&](int k) { return slot[k * kFeatBlock]; }, posrho, velp, dt_mult,
+                      inv_v);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1188,9 +1252,15 @@ hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows,
                                    float v_ref, uint16_t* X, bool lists, hipStream_t s) {
     if (nrows == 0) return hipSuccess;
     if (lists && (!b.nlist || !b.ncount)) return hipErrorInvalidValue;
+    if (lists) {
+        hipLaunchKernelGGL(k_closure_features_short, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, b.sc, b.posrho[1],
+                           b.velp[1], rows, nrows, dt_mult, 1.0f / v_ref, X, b.nlist, b.ncount);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(nrows, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc,
                        b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, 1.0f / v_ref, X,
-                       lists ? b.nlist : nullptr, lists ? b.ncount : nullptr);
+                       lists ? b.nlist : nullptr, lists ? b.ncount : nullptr, lists);
     return hipGetLastError();
 }
 
