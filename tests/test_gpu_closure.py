@@ -149,6 +149,25 @@ def test_features_match_numpy_reference(scene, lists, oracle):
         assert np.array_equal(got[r], _bf16_bits(want)), f"row {r} (slot {t}) differs"
 
 
+def test_closure_run_identical_with_and_without_lists():
+    """A learned run with a random network: with neighbour lists the macro step's closure rows come
+    from the lists (short lists straight, longer ones by selection), without them from the
+    searching feature kernel; the trajectories and the judge's statistics must be identical."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    K, macro = 4, 6
+    net = ClosureNet(max_batch=64)
+    net.set_params((np.random.default_rng(11).normal(0, 0.05, ClosureNet.num_params())).astype(np.float32))
+    res = []
+    for lists in (0, 1):
+        with Fluid(p, parts) as f:
+            f.set_tuning(2, 2, lists)
+            st = L.wc_closure_stats()
+            L.check(L.lib().wc_closure_run(f.ctx, net.h, C.c_float(dt), K, macro, C.c_float(0.01), C.byref(st)),
+                    "wc_closure_run")
+            res.append((f.particles(1).tobytes(), st.macro_steps, st.fallbacks, st.max_density_jump))
+    assert res[0] == res[1]
+
+
 def _zero_net():
     net = ClosureNet(max_batch=64)
     net.set_params(np.zeros(ClosureNet.num_params(), np.float32))
