@@ -340,6 +340,11 @@ WC_API int wc_max_density(wc_ctx* ctx, float* out);
  * With neighbour lists (wc_tuning.neighbour_list) the neighbours are read from the lists of the
  * last density pass, which must follow the last sort. */
 WC_API int wc_closure_features(wc_ctx* ctx, float dt_mult, const uint32_t* rows, uint32_t nrows, void* x_dev);
+/* The network on all n closure inputs of buffer 2 without forming the feature matrix: each row's
+ * values are formed inside the inference kernel from the neighbour lists (required) of the last
+ * density pass.  y_dev receives n rows of 4 floats, the same as wc_closure_features followed by
+ * wc_mlp_predict. */
+WC_API int wc_closure_infer(wc_ctx* ctx, wc_mlp* net, float dt_mult, void* y_dev);
 /* Force pass buffer 2 -> buffer 1 with step dt; corr_dev (n rows of 4 floats, times corr_scale)
  * is added to the acceleration of particles at least 2h from every wall (NULL: none); acc_dev
  * (NULL: not written) receives each slot's plain acceleration (4th float: 1 where a correction

@@ -149,6 +149,37 @@ def test_features_match_numpy_reference(scene, lists, oracle):
         assert np.array_equal(got[r], _bf16_bits(want)), f"row {r} (slot {t}) differs"
 
 
+@pytest.mark.parametrize("scene", ["dam_break_16k", "dense_h3"])
+def test_inference_from_lists_equals_features_then_network(scene):
+    """wc_closure_infer forms each row's values inside the inference kernel (short lists read
+    straight, longer ones through the selection's slots); it must give the bits of
+    wc_closure_features followed by wc_mlp_predict.  dense_h3's lists all overflow."""
+    if scene == "dam_break_16k":
+        p, parts, dt, _ = scenes.workload("dam_break_16k")
+    else:
+        p, parts = scenes.scaled_block((24, 24, 24), 0.01, seed=5, h_over_s=3.0)
+        dt = 2e-4
+    n = p.num_particles
+    print(f"closure inference from lists {scene}: {n} particles", flush=True)
+    net = ClosureNet(max_batch=64)
+    net.set_params((np.random.default_rng(12).normal(0, 0.05, ClosureNet.num_params())).astype(np.float32))
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, 1)
+        f.step(dt, 30)
+        f.sort_.run()
+        f.runDensityProg()
+        x = DeviceArray((n, 208), np.uint16)
+        y1 = DeviceArray((n, 4), np.float32)
+        y2 = DeviceArray((n, 4), np.float32)
+        L.check(L.lib().wc_closure_infer(f.ctx, net.h, C.c_float(4.0), y1.ptr), "wc_closure_infer")
+        L.check(L.lib().wc_closure_features(f.ctx, C.c_float(4.0), None, 0, x.ptr), "wc_closure_features")
+        L.check(L.lib().wc_mlp_predict(net.h, x.ptr, y2.ptr, n), "wc_mlp_predict")
+        a, b = y1.download(), y2.download()
+    bad = np.nonzero((a.view(np.uint32) != b.view(np.uint32)).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
+    assert np.abs(a).max() > 0
+
+
 def test_closure_run_identical_with_and_without_lists():
     """A learned run with a random network: with neighbour lists the macro step's closure rows come
     from the lists (short lists straight, longer ones by selection), without them from the

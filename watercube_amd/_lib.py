@@ -171,6 +171,7 @@ SIGNATURES = {
     "wc_get_ids": (C.c_int, [_vp, C.c_int, _u32p, C.c_uint32]),
     "wc_max_density": (C.c_int, [_vp, _f32p]),
     "wc_closure_features": (C.c_int, [_vp, C.c_float, _u32p, C.c_uint32, _vp]),
+    "wc_closure_infer": (C.c_int, [_vp, _vp, C.c_float, _vp]),
     "wc_closure_update": (C.c_int, [_vp, C.c_float, _vp, C.c_float, _vp]),
     "wc_closure_set_smoothing": (C.c_int, [_vp, C.c_int]),
     "wc_closure_set_integrator": (C.c_int, [_vp, C.c_int]),

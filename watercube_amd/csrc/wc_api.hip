@@ -1256,6 +1256,33 @@ WC_API int wc_closure_features(wc_ctx* c, float dt_mult, const uint32_t* rows_ho
     return WC_OK;
 }
 
+// Inference on the closure inputs formed from the neighbour lists (no feature matrix): the
+// selection of longer lists into cl_x's memory, then the network.
+static hipError_t closure_infer_from_lists(wc_ctx* c, wc_mlp* net, float dt_mult, float* y) {
+    const uint32_t n = c->total_particles;
+    hipError_t e = wc::launch_closure_selection(c->b, n, reinterpret_cast<uint32_t*>(c->cl_x), c->stream);
+    if (e != hipSuccess) return e;
+    const wc::StepConsts& k = c->desc[0].c;
+    const wc::ClosureRows rows{c->b.posrho[1], c->b.velp[1], c->b.nlist, c->b.ncount,
+                               reinterpret_cast<const uint32_t*>(c->cl_x), (size_t)c->total_particles, k.h, k.rho0,
+                               dt_mult, 1.0f};
+    return wc::mlp_infer_closure(net->m, rows, y, n, c->stream);
+}
+
+WC_API int wc_closure_infer(wc_ctx* c, wc_mlp* net, float dt_mult, void* y_dev) {
+    if (!c || !net || !y_dev) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "the learned closure runs on single-scene contexts");
+    if (!c->neighbour_list) return fail(WC_ERR_INVALID_ARG, "closure inputs from the lists: enable neighbour lists");
+    int rc = closure_workspace(c);
+    if (rc) return rc;
+    rc = set_device(c);
+    if (rc) return rc;
+    WC_HIP(hipStreamSynchronize(net->m.stream));
+    WC_HIP(closure_infer_from_lists(c, net, dt_mult, (float*)y_dev));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    return WC_OK;
+}
+
 WC_API int wc_closure_update(wc_ctx* c, float dt, const void* corr_dev, float corr_scale, void* acc_dev) {
     if (!c) return fail(WC_ERR_INVALID_ARG, "ctx is NULL");
     if (c->num_scenes != 1) return fail(WC_ERR_INVALID_ARG, "the learned closure runs on single-scene contexts");
@@ -1350,7 +1377,9 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
                               c->stream));
         if (c->b.ids[0])
             WC_HIP(hipMemcpyAsync(c->cl_backup_ids, c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
-        if (net) {
+        if (net && c->neighbour_list) {
+            WC_HIP(closure_infer_from_lists(c, net, (float)K, c->cl_y));
+        } else if (net) {
             WC_HIP(wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->neighbour_list,
                                                c->stream));
             WC_HIP(wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream));

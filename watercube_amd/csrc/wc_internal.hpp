@@ -134,6 +134,10 @@ hipError_t launch_place(const DeviceBuffers& b, hipStream_t stream);
 hipError_t launch_gather(const DeviceBuffers& b, hipStream_t stream);
 // With `list`, the density pass also writes the neighbour lists and the force pass reads them.
 hipError_t launch_density(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
+// For the single scene's particles with more than kClosureNeighbours listed neighbours (or an
+// overflowed list): the slots of the kClosureNeighbours nearest, in list order, at
+// sel[t * kClosureFeatures / 2 + k].  Other rows are left alone.
+hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t* sel, hipStream_t stream);
 hipError_t launch_update(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
 hipError_t launch_advect(const DeviceBuffers& b, float dt, hipStream_t stream);
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n,

@@ -1045,7 +1045,7 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
                                                                  float dt_mult, float inv_v, uint16_t* __restrict__ X,
                                                                  const uint32_t* __restrict__ nlist,
                                                                  const uint32_t* __restrict__ ncount,
-                                                                 bool skip_short) {
+                                                                 bool skip_short, uint32_t* __restrict__ sel_out) {
     static_assert(kFeatChunk <= (int)kRecordPad + 1, "chunked loads may run kFeatChunk - 1 records past a cell");
     static_assert(kFeatCap >= kClosureNeighbours + kFeatChunk, "a chunk must fit after a selection");
     __shared__ uint32_t s_key[kFeatCap * kFeatBlock];
@@ -1055,9 +1055,10 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
     const SceneDesc& d = S.desc[0];
     const StepConsts& c = d.c;
     const uint32_t t = rows ? rows[r] : d.p_off + r;
-    uint4* row = reinterpret_cast<uint4*>(X + (size_t)r * kClosureFeatures);
+    uint4* row = X ? reinterpret_cast<uint4*>(X + (size_t)r * kClosureFeatures) : nullptr;  // null: slots to sel_out
     if (t - d.p_off >= c.n) {
-        for (int q = 0; q < kClosureFeatures / 8; ++q) row[q] = make_uint4(0u, 0u, 0u, 0u);
+        if (row)
+            for (int q = 0; q < kClosureFeatures / 8; ++q) row[q] = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
     // rows of short lists are written by k_closure_features_short
@@ -1115,6 +1116,13 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
     }
     if (cnt > kClosureNeighbours) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
 
+    if (sel_out) {
+        // the kept neighbours' slots instead of the row (row r's words of X), for inference that
+        // forms the values itself
+        uint32_t* o = sel_out + (size_t)r * (kClosureFeatures / 2);
+        for (int k = 0; k < cnt; ++k) o[k] = slot[k * kFeatBlock];
+        return;
+    }
     write_feature_row(row, c, t, p, pv, cnt, [&](int k) { return slot[k * kFeatBlock]; }, posrho, velp, dt_mult,
                       inv_v);
 }
@@ -1260,7 +1268,15 @@ hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows,
     }
     hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(nrows, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc,
                        b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, 1.0f / v_ref, X,
-                       lists ? b.nlist : nullptr, lists ? b.ncount : nullptr, lists);
+                       lists ? b.nlist : nullptr, lists ? b.ncount : nullptr, lists, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t* sel, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (!b.nlist || !b.ncount || !sel) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(n, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc, b.posrho[1],
+                       b.velp[1], b.skey, b.range, nullptr, n, 0.0f, 1.0f, nullptr, b.nlist, b.ncount, true, sel);
     return hipGetLastError();
 }
 

@@ -250,7 +250,7 @@ __device__ __forceinline__ void load_rows(uint4 (&a)[kKSteps1], const uint16_t* 
 
 // act_out = bf16(silu(A W + b)) for a wave's 16 rows; W^T rows from LDS with stride LDW; A
 // operands from `areg` (KSTEPS registers, layer 1) or from the wave's activation buffer.
-template <int KSTEPS, int LDW, bool FROM_REGS>
+template <int KSTEPS, int LDW, bool FROM_REGS, bool LEAN = false>
 __device__ __forceinline__ void infer_layer(const uint4* areg, const uint16_t* act_in, const uint16_t* __restrict__ wt,
                                             const float* __restrict__ bias, uint16_t* act_out, int lane) {
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
@@ -261,7 +261,9 @@ __device__ __forceinline__ void infer_layer(const uint4* areg, const uint16_t* a
         const float bj = bias[j * 16 + r16];
         acc[j] = f32x4{bj, bj, bj, bj};
     }
-#pragma unroll
+    // LEAN (operands from LDS only): one k step's weights in flight at a time, leaving registers to
+    // the caller
+#pragma unroll(LEAN ? 1 : KSTEPS)
     for (int s = 0; s < KSTEPS; ++s) {
         const uint4 av = FROM_REGS ? areg[s] : *reinterpret_cast<const uint4*>(&act_in[r16 * kLdH + 32 * s + kq]);
 #pragma unroll
@@ -342,6 +344,180 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(const uint16_t* 
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Inference on the closure's inputs without the feature matrix: each lane forms its own layer-1
+// operand from the neighbour lists.  Lane l (row l % 16, quarter q = l / 16) holds columns
+// 32 s + 8 q .. + 7 of k-step s; for s < 6 these are six values of neighbour k0 = (32 s + 8 q) / 6
+// from offset o = 32 s + 8 q - 6 k0 (0, 2 or 4), then the first of neighbour k0 + 1, so every
+// step needs two neighbours.  The next tile's rows are fetched in three phases between the
+// current tile's layers (list entries, then records, then the values), so the loads land
+// while the matrix cores work.
+// ------------------------------------------------------------------------------------------
+constexpr int kFeatSteps = 6;  // k-steps of neighbour values (192 columns)
+
+struct RowFetch {
+    uint32_t row, listed;
+    bool live;
+    float4 p;   // own position and density
+    float3 pv;  // own velocity
+    uint32_t e_list[kFeatSteps][2], e_sel[kFeatSteps][2];  // phase 1: both candidates for each entry
+    float3 q[kFeatSteps][2], qv[kFeatSteps][2];            // phase 2: the neighbours' records
+};
+
+__device__ __forceinline__ uint32_t pack2bf(float lo, float hi) {
+    typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
+    typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+    const f32x2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+__device__ __forceinline__ float3 load3(const float4* a, uint32_t i) {
+    const float* f = reinterpret_cast<const float*>(a) + 4 * (size_t)i;
+    return float3{f[0], f[1], f[2]};
+}
+
+// phase 1: the row's list entries and own record.  A row of more than 32 neighbours takes the 32
+// slots its selection wrote (cr.sel); both candidates are loaded so that no load waits for the
+// count.
+__device__ __forceinline__ void fetch_entries(RowFetch& f, const ClosureRows& cr, long row0, uint32_t n, int lane) {
+    const int r16 = lane & 15, q4 = lane >> 4;
+    f.live = row0 + r16 < (long)n;
+    f.row = f.live ? (uint32_t)(row0 + r16) : 0u;
+    f.listed = f.live ? cr.ncount[f.row] : 0u;
+    f.p = cr.posrho[f.row];
+    f.pv = load3(cr.velp, f.row);
+#pragma unroll
+    for (int s = 0; s < kFeatSteps; ++s) {
+        const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            f.e_list[s][j] = cr.nlist[(size_t)(k0 + j) * cr.stride + f.row];
+            f.e_sel[s][j] = cr.sel[(size_t)f.row * kSelStride + k0 + j];
+        }
+    }
+}
+
+// phase 2: the neighbours' records (entries past a list may be stale: a slot out of range is
+// replaced by the row's own, so every load stays inside the buffers)
+__device__ __forceinline__ void fetch_records(RowFetch& f, const ClosureRows& cr, uint32_t n) {
+    const bool sel = f.listed > 32u;
+#pragma unroll
+    for (int s = 0; s < kFeatSteps; ++s)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            uint32_t u = sel ? f.e_sel[s][j] : f.e_list[s][j];
+            u = u < n ? u : f.row;
+            f.q[s][j] = load3(cr.posrho, u);
+            f.qv[s][j] = load3(cr.velp, u);
+        }
+}
+
+// phase 3: the layer-1 operand, formed as the feature kernel forms its rows (same operations, so
+// the same bf16 values)
+__device__ __forceinline__ void form_rows(uint4 (&a)[kKSteps1], const RowFetch& f, const ClosureRows& cr, int lane) {
+    const int q4 = lane >> 4;
+    if (!f.live) {
+#pragma unroll
+        for (int s = 0; s < kKSteps1; ++s) a[s] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const uint32_t cnt = f.listed > 32u ? 32u : f.listed;  // longer lists keep their 32 nearest
+    const float inv_h = 1.0f / cr.h;
+#pragma unroll
+    for (int s = 0; s < kFeatSteps; ++s) {
+        const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
+        const uint32_t o = (uint32_t)(32 * s + 8 * q4) - 6u * k0;  // 0, 2 or 4
+        float v[12];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            // zeros past the list by masking the bits (an opaque mask keeps the loads unconditional)
+            uint32_t m = k0 + (uint32_t)j < cnt ? ~0u : 0u;
+            asm("" : "+v"(m));
+            auto keep = [m](float x) { return __uint_as_float(__float_as_uint(x) & m); };
+            v[6 * j + 0] = keep((f.q[s][j].x - f.p.x) * inv_h);
+            v[6 * j + 1] = keep((f.q[s][j].y - f.p.y) * inv_h);
+            v[6 * j + 2] = keep((f.q[s][j].z - f.p.z) * inv_h);
+            v[6 * j + 3] = keep((f.qv[s][j].x - f.pv.x) * cr.inv_v);
+            v[6 * j + 4] = keep((f.qv[s][j].y - f.pv.y) * cr.inv_v);
+            v[6 * j + 5] = keep((f.qv[s][j].z - f.pv.z) * cr.inv_v);
+        }
+        float c[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) c[e] = o == 0u ? v[e] : (o == 2u ? v[e + 2] : v[e + 4]);
+        a[s] = make_uint4(pack2bf(c[0], c[1]), pack2bf(c[2], c[3]), pack2bf(c[4], c[5]), pack2bf(c[6], c[7]));
+    }
+    // columns 192..207: the density deviation, the step multiplier and the filled fraction
+    const float rho_rel = (f.p.w / cr.rho0 - 1.0f) * 10.0f;
+    a[kFeatSteps] = q4 == 0 ? make_uint4(pack2bf(rho_rel, cr.dt_mult * 0.25f),
+                                         pack2bf((float)cnt * (1.0f / 32.0f), 0.0f), 0u, 0u)
+                            : make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer_closure(ClosureRows cr, InferWeights W,
+                                                                         float* __restrict__ Y, uint32_t n) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_mem[kInferLdsHalves];
+    uint16_t* w1 = s_mem;
+    uint16_t* w2 = w1 + kMlpHidden * kLd1;
+    uint16_t* w3 = w2 + kMlpHidden * kLdH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint16_t* act = w3 + kMlpHidden * kLdH + wave * kInferRows * kLdH;
+
+    const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
+    uint4 xa[kKSteps1];
+    RowFetch f;
+    if (blockIdx.x < tiles) {
+        fetch_entries(f, cr, (long)blockIdx.x * kInferTile + wave * kInferRows, n, lane);
+        fetch_records(f, cr, n);
+        form_rows(xa, f, cr, lane);
+    }
+    for (int i = tid; i < kMlpHidden * (kLd1 / 8); i += kInferWaves * 64) {
+        const int r = i / (kLd1 / 8), q = i % (kLd1 / 8);
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (q < kMlpInPad / 8) v = reinterpret_cast<const uint4*>(W.w1t)[r * (kMlpInPad / 8) + q];
+        *reinterpret_cast<uint4*>(&w1[r * kLd1 + q * 8]) = v;
+    }
+    for (int i = tid; i < kMlpHidden * (kMlpHidden / 8); i += kInferWaves * 64) {
+        const int r = i / (kMlpHidden / 8), q = i % (kMlpHidden / 8);
+        *reinterpret_cast<uint4*>(&w2[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w2t)[i];
+        *reinterpret_cast<uint4*>(&w3[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w3t)[i];
+    }
+    __syncthreads();
+
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const long row0 = (long)tile * kInferTile + wave * kInferRows;
+        const int rows_left = (int)max(0L, min((long)kInferRows, (long)n - row0));
+        const uint32_t next = tile + gridDim.x;
+        const bool more = next < tiles;
+        infer_layer<kKSteps1, kLd1, true>(xa, nullptr, w1, W.b1, act, lane);
+        if (more) fetch_entries(f, cr, (long)next * kInferTile + wave * kInferRows, n, lane);
+        wave_lds_sync();
+        infer_layer<kMlpHidden / 32, kLdH, false, true>(nullptr, act, w2, W.b2, act, lane);
+        if (more) fetch_records(f, cr, n);
+        wave_lds_sync();
+        infer_layer<kMlpHidden / 32, kLdH, false, true>(nullptr, act, w3, W.b3, act, lane);
+        wave_lds_sync();
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int r16 = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+        for (int s = 0; s < kMlpHidden / 32; ++s) {
+            const uint4 av = *reinterpret_cast<const uint4*>(&act[r16 * kLdH + 32 * s + kq]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&W.w4t[(long)r16 * kMlpHidden + 32 * s + kq]);
+            acc = mfma16(av, bv, acc);
+        }
+        const int col = r16;
+        if (col < 4) {
+            const float bc = col < kMlpOut ? W.b4[col] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = 4 * (lane >> 4) + r;
+                if (rr < rows_left) Y[(row0 + rr) * 4 + col] = col < kMlpOut ? acc[r] + bc : 0.0f;
+            }
+        }
+        if (more) form_rows(xa, f, cr, lane);
+        wave_lds_sync();
+    }
+}
+
 inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -410,6 +586,16 @@ hipError_t mlp_infer(const Mlp& m, const uint16_t* X, float* Y, uint32_t n, hipS
     // one workgroup per CU (the kernel holds most of a CU's LDS), each walking several tiles
     const unsigned tiles = nblk(n, kInferTile);
     hipLaunchKernelGGL(k_mlp_infer, dim3(tiles < 256u ? tiles : 256u), dim3(kInferWaves * 64), 0, s, X, W, Y, n);
+    return hipGetLastError();
+}
+
+hipError_t mlp_infer_closure(const Mlp& m, const ClosureRows& rows, float* Y, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const MlpLayout L = mlp_layout();
+    InferWeights W{m.w1t, m.w2t, m.w3t, m.w4t, m.master + L.b1, m.master + L.b2, m.master + L.b3, m.master + L.b4};
+    const unsigned tiles = nblk(n, kInferTile);
+    hipLaunchKernelGGL(k_mlp_infer_closure, dim3(tiles < 256u ? tiles : 256u), dim3(kInferWaves * 64), 0, s, rows, W,
+                       Y, n);
     return hipGetLastError();
 }
 

@@ -46,6 +46,21 @@ hipError_t gemm(const uint16_t* A, long sam, long sak, const uint16_t* B, long s
                 int N, int K, bool accumulate, hipStream_t s);
 hipError_t mlp_refresh_bf16(Mlp& m, hipStream_t s);
 hipError_t mlp_infer(const Mlp& m, const uint16_t* X, float* Y, uint32_t n, hipStream_t s);
+// The learned closure's inputs, for inference that forms each row's layer-1 operand itself, the
+// values the closure's feature kernel would write: from a particle's neighbour list when it
+// holds at most 32, else from the 32 nearest its selection wrote to `sel`.  Single scene, row r =
+// slot r.
+constexpr int kSelStride = kMlpInPad / 2;  // 32-bit words per row of the selection buffer
+struct ClosureRows {
+    const float4* posrho;    // sorted records: position, density
+    const float4* velp;      // velocity, pressure
+    const uint32_t* nlist;   // entry k of slot t at nlist[k * stride + t]
+    const uint32_t* ncount;  // listed neighbours of slot t
+    const uint32_t* sel;     // more than 32 listed: the 32 nearest, in list order, at sel[t * kSelStride + k]
+    size_t stride;
+    float h, rho0, dt_mult, inv_v;
+};
+hipError_t mlp_infer_closure(const Mlp& m, const ClosureRows& rows, float* Y, uint32_t n, hipStream_t s);
 hipError_t mlp_forward_train(Mlp& m, const uint16_t* X, const float* T, float* Y, uint32_t M, hipStream_t s);
 hipError_t mlp_backward(Mlp& m, const uint16_t* X, uint32_t M, hipStream_t s);
 hipError_t mlp_adam(Mlp& m, float lr, hipStream_t s);
