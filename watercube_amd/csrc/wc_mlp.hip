@@ -263,7 +263,8 @@ __device__ __forceinline__ void infer_layer(const uint4* areg, const uint16_t* a
     }
     // LEAN (operands from LDS only): one k step's weights in flight at a time, leaving registers to
     // the caller
-#pragma unroll(LEAN ? 1 : KSTEPS)
+    constexpr int kUnrollSteps = LEAN ? 1 : KSTEPS;
+#pragma unroll kUnrollSteps
     for (int s = 0; s < KSTEPS; ++s) {
         const uint4 av = FROM_REGS ? areg[s] : *reinterpret_cast<const uint4*>(&act_in[r16 * kLdH + 32 * s + kq]);
 #pragma unroll
