@@ -137,8 +137,9 @@ struct wc_ctx {
     // learned-closure workspace (allocated on first use)
     uint16_t* cl_x = nullptr;       // features, n x 208 bf16
     float* cl_y = nullptr;          // network output, n x 4
-    float4* cl_backup[2] = {nullptr, nullptr};
-    uint32_t* cl_backup_ids = nullptr;
+    float4* cl_backup[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // two slots of (posrho, velp)
+    uint32_t* cl_backup_ids[2] = {nullptr, nullptr};
+    uint32_t* cl_host_rho = nullptr;  // pinned: the density maximum after each slot's learned step
     uint32_t* cl_rows = nullptr;    // row list for feature requests
     uint32_t cl_rows_cap = 0;
     bool cl_smooth = true;          // average the correction over neighbours in wc_closure_run
@@ -223,17 +224,20 @@ void free_ctx_buffers(wc_ctx* c) {
                     b.slot_tmp,    b.sorted,      b.skey,        b.counts,      b.range,     b.scan_status,
                     b.scan_ticket, b.error_flag,  b.scene_done,  b.step_start,  c->staging,  c->d_desc,
                     c->d_tables,   b.rho_max,     b.ids[0],      b.ids[1],      c->cl_x,     c->cl_y,
-                    c->cl_backup[0], c->cl_backup[1], c->cl_backup_ids, c->cl_rows, b.nlist, b.ncount};
+                    c->cl_backup[0][0], c->cl_backup[0][1], c->cl_backup[1][0], c->cl_backup[1][1],
+                    c->cl_backup_ids[0], c->cl_backup_ids[1], c->cl_rows, b.nlist, b.ncount};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->cl_host_rho) (void)hipHostFree(c->cl_host_rho);
+    c->cl_host_rho = nullptr;
     memset(&b, 0, sizeof(b));
     c->staging = nullptr;
     c->d_desc = nullptr;
     c->d_tables = nullptr;
     c->cl_x = nullptr;
     c->cl_y = nullptr;
-    c->cl_backup[0] = c->cl_backup[1] = nullptr;
-    c->cl_backup_ids = nullptr;
+    c->cl_backup[0][0] = c->cl_backup[0][1] = c->cl_backup[1][0] = c->cl_backup[1][1] = nullptr;
+    c->cl_backup_ids[0] = c->cl_backup_ids[1] = nullptr;
     c->cl_rows = nullptr;
 }
 
@@ -1222,8 +1226,14 @@ static int closure_workspace(wc_ctx* c) {
     };
     alloc((void**)&c->cl_x, n * wc::kClosureFeatures * 2);
     alloc((void**)&c->cl_y, n * 16);
-    alloc((void**)&c->cl_backup[0], n * sizeof(float4));
-    alloc((void**)&c->cl_backup[1], n * sizeof(float4));
+    for (int slot = 0; slot < 2; ++slot) {
+        alloc((void**)&c->cl_backup[slot][0], n * sizeof(float4));
+        alloc((void**)&c->cl_backup[slot][1], n * sizeof(float4));
+    }
+    if (ok && !c->cl_host_rho && hipHostMalloc((void**)&c->cl_host_rho, 2 * sizeof(uint32_t)) != hipSuccess) {
+        c->cl_host_rho = nullptr;
+        ok = false;
+    }
     if (!ok) return fail(WC_ERR_NO_MEMORY, "closure workspace for %zu particles", n);
     return WC_OK;
 }
@@ -1328,55 +1338,31 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
     const float rho0 = c->desc[0].c.rho0;
     const float dt_large = dt * (float)K;
     const float g = 9.81f;
-    if (c->b.ids[0] && !c->cl_backup_ids && hipMalloc(&c->cl_backup_ids, (size_t)(n ? n : 1) * 4) != hipSuccess)
-        return fail(WC_ERR_NO_MEMORY, "identity backup");
+    for (int slot = 0; slot < 2; ++slot)
+        if (c->b.ids[0] && !c->cl_backup_ids[slot] &&
+            hipMalloc(&c->cl_backup_ids[slot], (size_t)(n ? n : 1) * 4) != hipSuccess)
+            return fail(WC_ERR_NO_MEMORY, "identity backup");
     EventPair ev;
     WC_HIP(ev.create());
     WC_HIP(hipEventRecord(ev.start, c->stream));
-    float prev_max = 0.0f;
-    // Judge a learned step from the density pass of its new state: compare the density maximum
-    // with the one before the step; past the tolerance, replace the step by K plain steps from
-    // the saved state.
     auto sort_density = [&]() -> int {
         int r = enqueue_sort(c);
         if (r) return r;
         WC_HIP(wc::launch_density(c->b, c->density_variant, c->neighbour_list, c->stream));
         return WC_OK;
     };
-    auto judge = [&](float* now) -> int {
-        int r = read_rho_max(c, now);
-        if (r) return r;
-        const float jump = (*now - prev_max) / rho0;
-        if ((double)jump > stats->max_density_jump) stats->max_density_jump = jump;
-        if (!(jump <= judge_tol)) {  // NaN fails too
-            stats->fallbacks += 1;
-            WC_HIP(hipMemcpyAsync(c->b.posrho[0], c->cl_backup[0], (size_t)n * sizeof(float4),
-                                  hipMemcpyDeviceToDevice, c->stream));
-            WC_HIP(hipMemcpyAsync(c->b.velp[0], c->cl_backup[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
-                                  c->stream));
-            if (c->b.ids[0])
-                WC_HIP(hipMemcpyAsync(c->b.ids[0], c->cl_backup_ids, (size_t)n * 4, hipMemcpyDeviceToDevice,
-                                      c->stream));
-            r = run_steps(c, K);
-            if (r) return r;
-            r = sort_density();
-            if (r) return r;
-            r = read_rho_max(c, now);
-            if (r) return r;
-        }
-        return WC_OK;
-    };
-    // One macro step without its judge's verdict: save the sorted state, form the features, run
-    // the network, take the large step, then sort and run the density pass of the new state (the
-    // judge's input, and the next step's).  Captured once per call and replayed, so the GPU does
-    // not wait on a dozen launches after each verdict.
-    auto enqueue_macro = [&]() -> int {
-        WC_HIP(hipMemcpyAsync(c->cl_backup[0], c->b.posrho[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
-                              c->stream));
-        WC_HIP(hipMemcpyAsync(c->cl_backup[1], c->b.velp[1], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice,
-                              c->stream));
+    // One macro step: save the sorted state into `slot`, form the closure inputs, run the network,
+    // take the large step, sort and run the density pass of the new state (the judge's input, and
+    // the next step's), and copy the density maximum to the host.  Captured once per slot and
+    // replayed.
+    auto enqueue_macro = [&](int slot) -> int {
+        WC_HIP(hipMemcpyAsync(c->cl_backup[slot][0], c->b.posrho[1], (size_t)n * sizeof(float4),
+                              hipMemcpyDeviceToDevice, c->stream));
+        WC_HIP(hipMemcpyAsync(c->cl_backup[slot][1], c->b.velp[1], (size_t)n * sizeof(float4),
+                              hipMemcpyDeviceToDevice, c->stream));
         if (c->b.ids[0])
-            WC_HIP(hipMemcpyAsync(c->cl_backup_ids, c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+            WC_HIP(hipMemcpyAsync(c->cl_backup_ids[slot], c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice,
+                                  c->stream));
         if (net && c->neighbour_list) {
             WC_HIP(closure_infer_from_lists(c, net, (float)K, c->cl_y));
         } else if (net) {
@@ -1386,40 +1372,86 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         }
         WC_HIP(wc::launch_closure_update(c->b, dt_large, net ? c->cl_y : nullptr, g, nullptr, c->cl_smooth,
                                          c->cl_substeps ? (float)K : 0.0f, c->neighbour_list, c->stream));
-        return sort_density();
+        int r = sort_density();
+        if (r) return r;
+        WC_HIP(hipMemcpyAsync(c->cl_host_rho + slot, c->b.rho_max, 4, hipMemcpyDeviceToHost, c->stream));
+        return WC_OK;
     };
-    struct MacroGraph {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        ~MacroGraph() {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
+    struct MacroGraphs {
+        hipStream_t stream = nullptr;
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
+        hipEvent_t done[2] = {nullptr, nullptr};
+        ~MacroGraphs() {
+            // a speculative step may still be queued when an error ends the run
+            if (stream) (void)hipStreamSynchronize(stream);
+            for (int i = 0; i < 2; ++i) {
+                if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+                if (graph[i]) (void)hipGraphDestroy(graph[i]);
+                if (done[i]) (void)hipEventDestroy(done[i]);
+            }
         }
     } mg;
+    mg.stream = c->stream;
     // the network's own stream may still be writing its bf16 weights
     if (net) WC_HIP(hipStreamSynchronize(net->m.stream));
+    float prev_max = 0.0f;
     rc = sort_density();
     if (!rc) rc = read_rho_max(c, &prev_max);
-    if (!rc && nmacro > 0) {
+    for (int slot = 0; slot < 2 && !rc && nmacro > 0; ++slot) {
+        WC_HIP(hipEventCreateWithFlags(&mg.done[slot], hipEventDisableTiming));
         WC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_macro();
-        const hipError_t e = hipStreamEndCapture(c->stream, &mg.graph);
+        rc = enqueue_macro(slot);
+        const hipError_t e = hipStreamEndCapture(c->stream, &mg.graph[slot]);
         if (!rc && e != hipSuccess) rc = fail(WC_ERR_HIP, "learned step capture: %s", hipGetErrorString(e));
         if (!rc) {
-            const hipError_t ei = hipGraphInstantiate(&mg.exec, mg.graph, nullptr, nullptr, 0);
+            const hipError_t ei = hipGraphInstantiate(&mg.exec[slot], mg.graph[slot], nullptr, nullptr, 0);
             if (ei != hipSuccess) rc = fail(WC_ERR_HIP, "learned step graph: %s", hipGetErrorString(ei));
         }
     }
+    // Macro step m runs in slot m % 2.  Step m + 1 is queued before step m's verdict is read, so
+    // the GPU does not wait on the host; it works on step m's result and saves it in the other
+    // slot, leaving slot m % 2 (the state before step m) intact.  When step m fails its judge, the
+    // state is restored from slot m % 2, K plain steps replace it, and step m + 1 is queued again
+    // after them: the speculative one ran on the rejected state and is overwritten.
+    auto launch = [&](uint32_t m) -> int {
+        WC_HIP(hipGraphLaunch(mg.exec[m % 2], c->stream));
+        WC_HIP(hipEventRecord(mg.done[m % 2], c->stream));
+        return WC_OK;
+    };
+    if (!rc && nmacro > 0) rc = launch(0);
     for (uint32_t m = 0; m < nmacro && !rc; ++m) {
-        const hipError_t e = hipGraphLaunch(mg.exec, c->stream);
+        const int slot = (int)(m % 2);
+        const bool ahead = m + 1 < nmacro;
+        if (ahead) {
+            rc = launch(m + 1);
+            if (rc) break;
+        }
+        const hipError_t e = hipEventSynchronize(mg.done[slot]);
         if (e != hipSuccess) {
             rc = fail(WC_ERR_HIP, "learned step: %s", hipGetErrorString(e));
             break;
         }
         stats->macro_steps += 1;
         float now = 0.0f;
-        rc = judge(&now);
-        if (!rc) prev_max = now;
+        memcpy(&now, c->cl_host_rho + slot, 4);
+        const float jump = (now - prev_max) / rho0;
+        if ((double)jump > stats->max_density_jump) stats->max_density_jump = jump;
+        if (!(jump <= judge_tol)) {  // NaN fails too
+            stats->fallbacks += 1;
+            WC_HIP(hipMemcpyAsync(c->b.posrho[0], c->cl_backup[slot][0], (size_t)n * sizeof(float4),
+                                  hipMemcpyDeviceToDevice, c->stream));
+            WC_HIP(hipMemcpyAsync(c->b.velp[0], c->cl_backup[slot][1], (size_t)n * sizeof(float4),
+                                  hipMemcpyDeviceToDevice, c->stream));
+            if (c->b.ids[0])
+                WC_HIP(hipMemcpyAsync(c->b.ids[0], c->cl_backup_ids[slot], (size_t)n * 4, hipMemcpyDeviceToDevice,
+                                      c->stream));
+            rc = run_steps(c, K);
+            if (!rc) rc = sort_density();
+            if (!rc) rc = read_rho_max(c, &now);
+            if (!rc && ahead) rc = launch(m + 1);
+        }
+        prev_max = now;
     }
     if (!rc) {
         hipError_t e = hipEventRecord(ev.stop, c->stream);

@@ -359,10 +359,11 @@ constexpr int kFeatSteps = 6;  // k-steps of neighbour values (192 columns)
 struct RowFetch {
     uint32_t row, listed;
     bool live;
-    float4 p;   // own position and density
-    float3 pv;  // own velocity
-    uint32_t e_list[kFeatSteps][2], e_sel[kFeatSteps][2];  // phase 1: both candidates for each entry
-    float3 q[kFeatSteps][2], qv[kFeatSteps][2];            // phase 2: the neighbours' records
+    uint32_t listed_next;  // the count of the row one tile further on, loaded a tile early
+    float4 p;              // own position and density
+    float3 pv;             // own velocity
+    uint32_t e[kFeatSteps][2];                  // phase 1: list entries (or the selection's slots)
+    float3 q[kFeatSteps][2], qv[kFeatSteps][2];  // phase 2: the neighbours' records
 };
 
 __device__ __forceinline__ uint32_t pack2bf(float lo, float hi) {
@@ -377,40 +378,53 @@ __device__ __forceinline__ float3 load3(const float4* a, uint32_t i) {
     return float3{f[0], f[1], f[2]};
 }
 
-// phase 1: the row's list entries and own record.  A row of more than 32 neighbours takes the 32
-// slots its selection wrote (cr.sel); both candidates are loaded so that no load waits for the
-// count.
+__device__ __forceinline__ uint32_t row_count(const ClosureRows& cr, long row0, uint32_t n, int lane) {
+    const long row = row0 + (lane & 15);
+    return row < (long)n ? cr.ncount[row] : 0u;
+}
+
+// phase 1: the row's list entries and own record; `listed` arrived a tile earlier, so the loads
+// go to the list itself or, past 32 neighbours, to the 32 slots its selection wrote (cr.sel)
 __device__ __forceinline__ void fetch_entries(RowFetch& f, const ClosureRows& cr, long row0, uint32_t n, int lane) {
     const int r16 = lane & 15, q4 = lane >> 4;
     f.live = row0 + r16 < (long)n;
     f.row = f.live ? (uint32_t)(row0 + r16) : 0u;
-    f.listed = f.live ? cr.ncount[f.row] : 0u;
+    f.listed = f.listed_next;
     f.p = cr.posrho[f.row];
     f.pv = load3(cr.velp, f.row);
+    const bool sel = f.listed > 32u;
 #pragma unroll
     for (int s = 0; s < kFeatSteps; ++s) {
         const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            f.e_list[s][j] = cr.nlist[(size_t)(k0 + j) * cr.stride + f.row];
-            f.e_sel[s][j] = cr.sel[(size_t)f.row * kSelStride + k0 + j];
-        }
+        for (int j = 0; j < 2; ++j)
+            f.e[s][j] = sel ? cr.sel[(size_t)f.row * kSelStride + k0 + j]
+                            : cr.nlist[(size_t)(k0 + j) * cr.stride + f.row];
     }
 }
 
-// phase 2: the neighbours' records (entries past a list may be stale: a slot out of range is
-// replaced by the row's own, so every load stays inside the buffers)
-__device__ __forceinline__ void fetch_records(RowFetch& f, const ClosureRows& cr, uint32_t n) {
-    const bool sel = f.listed > 32u;
+// phase 2: the neighbours' records, only the halves the lane's columns use (entries past a list
+// may be stale: a slot out of range is replaced by the row's own, so every load stays inside the
+// buffers)
+__device__ __forceinline__ void fetch_records(RowFetch& f, const ClosureRows& cr, uint32_t n, int lane) {
+    const int q4 = lane >> 4;
 #pragma unroll
-    for (int s = 0; s < kFeatSteps; ++s)
+    for (int s = 0; s < kFeatSteps; ++s) {
+        const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
+        const uint32_t o = (uint32_t)(32 * s + 8 * q4) - 6u * k0;  // 0, 2 or 4
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            uint32_t u = sel ? f.e_sel[s][j] : f.e_list[s][j];
+            uint32_t u = f.e[s][j];
             u = u < n ? u : f.row;
-            f.q[s][j] = load3(cr.posrho, u);
-            f.qv[s][j] = load3(cr.velp, u);
+            // neighbour k0 uses values o..5, neighbour k0 + 1 values 0..o + 1
+            const bool need_pos = j == 0 ? o <= 2u : true;
+            const bool need_vel = j == 0 ? true : o >= 2u;
+            f.q[s][j] = float3{0.0f, 0.0f, 0.0f};
+            f.qv[s][j] = float3{0.0f, 0.0f, 0.0f};
+            if (need_pos) f.q[s][j] = load3(cr.posrho, u);
+            if (need_vel) f.qv[s][j] = load3(cr.velp, u);
         }
+    }
 }
 
 // phase 3: the layer-1 operand, formed as the feature kernel forms its rows (same operations, so
@@ -467,8 +481,11 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer_closure(ClosureR
     uint4 xa[kKSteps1];
     RowFetch f;
     if (blockIdx.x < tiles) {
+        f.listed_next = row_count(cr, (long)blockIdx.x * kInferTile + wave * kInferRows, n, lane);
         fetch_entries(f, cr, (long)blockIdx.x * kInferTile + wave * kInferRows, n, lane);
-        fetch_records(f, cr, n);
+        const uint32_t second = blockIdx.x + gridDim.x;
+        f.listed_next = second < tiles ? row_count(cr, (long)second * kInferTile + wave * kInferRows, n, lane) : 0u;
+        fetch_records(f, cr, n, lane);
         form_rows(xa, f, cr, lane);
     }
     for (int i = tid; i < kMlpHidden * (kLd1 / 8); i += kInferWaves * 64) {
@@ -490,10 +507,14 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer_closure(ClosureR
         const uint32_t next = tile + gridDim.x;
         const bool more = next < tiles;
         infer_layer<kKSteps1, kLd1, true>(xa, nullptr, w1, W.b1, act, lane);
-        if (more) fetch_entries(f, cr, (long)next * kInferTile + wave * kInferRows, n, lane);
+        if (more) {
+            fetch_entries(f, cr, (long)next * kInferTile + wave * kInferRows, n, lane);
+            const uint32_t after = next + gridDim.x;
+            f.listed_next = after < tiles ? row_count(cr, (long)after * kInferTile + wave * kInferRows, n, lane) : 0u;
+        }
         wave_lds_sync();
         infer_layer<kMlpHidden / 32, kLdH, false, true>(nullptr, act, w2, W.b2, act, lane);
-        if (more) fetch_records(f, cr, n);
+        if (more) fetch_records(f, cr, n, lane);
         wave_lds_sync();
         infer_layer<kMlpHidden / 32, kLdH, false, true>(nullptr, act, w3, W.b3, act, lane);
         wave_lds_sync();
