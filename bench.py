@@ -233,14 +233,19 @@ def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int =
             if isinstance(e, dict) and "mean_density_err" in e:
                 cands.append((m, K, e))
 
-    def best(stat):
-        ok = [c for c in cands if c[2][stat] <= 0.01]
+    def best(stat, learned_only=False):
+        ok = [c for c in cands if c[2][stat] <= 0.01 and (not learned_only or c[0].startswith("learned"))]
+        if not ok:
+            return None
         m, K, e = max(ok, key=lambda c: c[2]["sim_s_per_s"])
         return {"method": m, "K": K, "particle_steps_per_s": e["sim_s_per_s"] / dt * n,
                 "speedup_vs_plain": e["speedup_vs_plain"], "mean_density_err": e["mean_density_err"],
                 "max_density_err": e["max_density_err"]}
 
-    res["best_within_1pct"] = {"by_mean_error": best("mean_density_err"), "by_max_error": best("max_density_err")}
+    # the fastest of all methods, and of the learned ones alone (the closure's target: at least 2x
+    # the plain dt run's simulated time per second within 1 % mean density error)
+    res["best_within_1pct"] = {"by_mean_error": best("mean_density_err"), "by_max_error": best("max_density_err"),
+                               "learned_by_mean_error": best("mean_density_err", learned_only=True)}
     return res
 
 
