@@ -1413,19 +1413,28 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
     // the GPU does not wait on the host; it works on step m's result and saves it in the other
     // slot, leaving slot m % 2 (the state before step m) intact.  When step m fails its judge, the
     // state is restored from slot m % 2, K plain steps replace it, and step m + 1 is queued again
-    // after them: the speculative one ran on the rejected state and is overwritten.
+    // after them: the speculative one ran on the rejected state and is overwritten.  A step is
+    // queued ahead only while the last density jump stayed under half the tolerance, so that a
+    // likely rejection does not also throw away the step after it.
     auto launch = [&](uint32_t m) -> int {
         WC_HIP(hipGraphLaunch(mg.exec[m % 2], c->stream));
         WC_HIP(hipEventRecord(mg.done[m % 2], c->stream));
         return WC_OK;
     };
-    if (!rc && nmacro > 0) rc = launch(0);
+    uint32_t queued = 0;  // macro steps launched so far
+    float last_jump = 0.0f;
     for (uint32_t m = 0; m < nmacro && !rc; ++m) {
         const int slot = (int)(m % 2);
-        const bool ahead = m + 1 < nmacro;
+        if (queued == m) {
+            rc = launch(m);
+            if (rc) break;
+            queued = m + 1;
+        }
+        const bool ahead = m + 1 < nmacro && !(last_jump > 0.5f * judge_tol);
         if (ahead) {
             rc = launch(m + 1);
             if (rc) break;
+            queued = m + 2;
         }
         const hipError_t e = hipEventSynchronize(mg.done[slot]);
         if (e != hipSuccess) {
@@ -1436,6 +1445,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         float now = 0.0f;
         memcpy(&now, c->cl_host_rho + slot, 4);
         const float jump = (now - prev_max) / rho0;
+        last_jump = jump;
         if ((double)jump > stats->max_density_jump) stats->max_density_jump = jump;
         if (!(jump <= judge_tol)) {  // NaN fails too
             stats->fallbacks += 1;
