@@ -348,11 +348,11 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(const uint16_t* 
 // ------------------------------------------------------------------------------------------
 // Inference on the closure's inputs without the feature matrix: each lane forms its own layer-1
 // operand from the neighbour lists.  Lane l (row l % 16, quarter q = l / 16) holds columns
-// 32 s + 8 q .. + 7 of k-step s; for s < 6 these are six values of neighbour k0 = (32 s + 8 q) / 6
-// from offset o = 32 s + 8 q - 6 k0 (0, 2 or 4), then the first of neighbour k0 + 1, so every
-// step needs two neighbours.  The next tile's rows are fetched in three phases between the
-// current tile's layers (list entries, then records, then the values), so the loads land
-// while the matrix cores work.
+// 32 s + 8 q .. + 7 of k-step s; for s < 6 these are values o..5 of neighbour
+// k0 = (32 s + 8 q) / 6, where o = 32 s + 8 q - 6 k0 is 0, 2 or 4, then values 0..o + 1 of
+// neighbour k0 + 1: two neighbours per step.  The next tile's rows are fetched in phases between
+// the current tile's layers (the row counts a tile earlier still, then the list entries, the
+// records, and last the values), so the loads land while the matrix cores work.
 // ------------------------------------------------------------------------------------------
 constexpr int kFeatSteps = 6;  // k-steps of neighbour values (192 columns)
 
