@@ -149,15 +149,19 @@ def test_features_match_numpy_reference(scene, lists, oracle):
         assert np.array_equal(got[r], _bf16_bits(want)), f"row {r} (slot {t}) differs"
 
 
-@pytest.mark.parametrize("scene", ["dam_break_16k", "dense_h3"])
+@pytest.mark.parametrize("scene", ["dam_break_16k", "dense_h3", "ragged"])
 def test_inference_from_lists_equals_features_then_network(scene):
     """wc_closure_infer forms each row's values inside the inference kernel (short lists read
     straight, longer ones through the selection's slots); it must give the bits of
-    wc_closure_features followed by wc_mlp_predict.  dense_h3's lists all overflow."""
+    wc_closure_features followed by wc_mlp_predict.  dense_h3's lists all overflow; ragged has
+    9,177 particles, so the last 128-row tile is partial."""
     if scene == "dam_break_16k":
         p, parts, dt, _ = scenes.workload("dam_break_16k")
-    else:
+    elif scene == "dense_h3":
         p, parts = scenes.scaled_block((24, 24, 24), 0.01, seed=5, h_over_s=3.0)
+        dt = 2e-4
+    else:
+        p, parts = scenes.scaled_block((23, 21, 19), 0.01, seed=7, h_over_s=2.0)
         dt = 2e-4
     n = p.num_particles
     print(f"closure inference from lists {scene}: {n} particles", flush=True)
@@ -180,11 +184,16 @@ def test_inference_from_lists_equals_features_then_network(scene):
     assert np.abs(a).max() > 0
 
 
-def test_closure_run_identical_with_and_without_lists():
+@pytest.mark.parametrize("scene", ["dam_break_16k", "ragged"])
+def test_closure_run_identical_with_and_without_lists(scene):
     """A learned run with a random network: with neighbour lists the macro step's closure rows come
     from the lists (short lists straight, longer ones by selection), without them from the
     searching feature kernel; the trajectories and the judge's statistics must be identical."""
-    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    if scene == "dam_break_16k":
+        p, parts, dt, _ = scenes.workload("dam_break_16k")
+    else:
+        p, parts = scenes.scaled_block((23, 21, 19), 0.01, seed=7, h_over_s=2.0)
+        dt = 2e-4
     K, macro = 4, 6
     net = ClosureNet(max_batch=64)
     net.set_params((np.random.default_rng(11).normal(0, 0.05, ClosureNet.num_params())).astype(np.float32))
