@@ -214,6 +214,18 @@ def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int =
             t0 = time.perf_counter()
             L.check(L.lib().wc_closure_features(f.ctx, C.c_float(4.0), None, n, x.ptr), "wc_closure_features")
             feat_s.append(time.perf_counter() - t0)
+    # the learned macro step's form: inputs formed inside the inference kernel from the lists
+    with Fluid(p, parts) as f:
+        f.set_tuning(4, 8, 1)
+        f.sort_.run()
+        f.runDensityProg()
+        f.synchronize()
+        fused_s = []
+        for _ in range(6):
+            t0 = time.perf_counter()
+            L.check(L.lib().wc_closure_infer(f.ctx, net.h, C.c_float(4.0), y.ptr), "wc_closure_infer")
+            fused_s.append(time.perf_counter() - t0)
+    fused_ms = 1e3 * float(np.median(fused_s[1:]))
     ms = net.profile_predict(x, y, n, reps=reps)
     x.free()
     y.free()
@@ -223,6 +235,9 @@ def learned_section(p, parts, dt, weights: Path, horizon: int = 100, reps: int =
                         "mfma_utilisation": tflops / MFMA_BF16_PEAK_TFLOPS,
                         "flop_per_particle": closure.mlp_flops_per_particle()}
     res["features_ms"] = 1e3 * float(np.median(feat_s))
+    fused_tflops = flops / (fused_ms * 1e-3) / 1e12
+    res["inference_from_lists"] = {"particles": n, "ms": fused_ms, "clock": "host, per call, selection included",
+                                   "tflops": fused_tflops, "mfma_utilisation": fused_tflops / MFMA_BF16_PEAK_TFLOPS}
     res["weights"] = str(weights.relative_to(ROOT)) if weights.is_relative_to(ROOT) else str(weights)
 
     # candidates: (method, K, entry); the plain small-dt run has zero error by definition
