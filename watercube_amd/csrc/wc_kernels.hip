@@ -752,30 +752,41 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
             q = posrho_in[u];
             qv = velp_in[u];
         };
+        // MODE 2: the neighbour's correction, loaded with its records (one neighbour ahead)
+        auto load_corr = [&](uint32_t u) {
+            if constexpr (MODE == 2) {
+                if (corr) return corr[u];
+            }
+            return float4{0.0f, 0.0f, 0.0f, 0.0f};
+        };
+        auto add_corr = [&](float dist, const float4& q, const float4& cj) {
+            if constexpr (MODE == 2) {
+                const float w = c.h - dist;
+                if (corr && closure_interior(c, q.x, q.y, q.z)) {
+                    csum.x += w * cj.x;
+                    csum.y += w * cj.y;
+                    csum.z += w * cj.z;
+                }
+                wsum += w;
+            }
+        };
         // neighbour k + 1's records are loaded while neighbour k's terms are formed
         auto drain = [&](int cnt) {
             if (cnt <= 0) return;
             uint32_t u = list[0];
             float4 q, qv;
             load(u, q, qv);
+            float4 cj = load_corr(u);
             for (int k = 0; k < cnt; ++k) {
                 const uint32_t un = list[(k + 1 < cnt ? k + 1 : k) * BLOCK];
                 float4 qn, qvn;
                 load(un, qn, qvn);
+                const float4 cjn = load_corr(un);
                 const float dist = f.template add<FAST>(c, p, pv, q, qv);
-                if constexpr (MODE == 2) {
-                    const float w = c.h - dist;
-                    if (corr && closure_interior(c, q.x, q.y, q.z)) {
-                        const float4 cj = corr[u];
-                        csum.x += w * cj.x;
-                        csum.y += w * cj.y;
-                        csum.z += w * cj.z;
-                    }
-                    wsum += w;
-                }
-                u = un;
+                add_corr(dist, q, cj);
                 q = qn;
                 qv = qvn;
+                cj = cjn;
             }
         };
         bool searched = true;
@@ -795,25 +806,17 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                     uint32_t u1 = nlist[(size_t)(last < 1u ? last : 1u) * stride + t];
                     float4 q, qv;
                     load(u0, q, qv);
-                    uint32_t u = u0;
+                    float4 cj = load_corr(u0);
                     for (uint32_t k = 0; k < cnt; ++k) {
                         const uint32_t u2 = nlist[(size_t)(k + 2 < cnt ? k + 2 : last) * stride + t];
                         float4 qn, qvn;
                         load(u1, qn, qvn);
+                        const float4 cjn = load_corr(u1);
                         const float dist = f.template add<FAST>(c, p, pv, q, qv);
-                        if constexpr (MODE == 2) {
-                            const float w = c.h - dist;
-                            if (corr && closure_interior(c, q.x, q.y, q.z)) {
-                                const float4 cj = corr[u];
-                                csum.x += w * cj.x;
-                                csum.y += w * cj.y;
-                                csum.z += w * cj.z;
-                            }
-                            wsum += w;
-                        }
-                        u = u1;
+                        add_corr(dist, q, cj);
                         q = qn;
                         qv = qvn;
+                        cj = cjn;
                         u1 = u2;
                     }
                 }
