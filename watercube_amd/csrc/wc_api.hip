@@ -1351,18 +1351,11 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         WC_HIP(wc::launch_density(c->b, c->density_variant, c->neighbour_list, c->stream));
         return WC_OK;
     };
-    // One macro step: save the sorted state into `slot`, form the closure inputs, run the network,
-    // take the large step, sort and run the density pass of the new state (the judge's input, and
-    // the next step's), and copy the density maximum to the host.  Captured once per slot and
-    // replayed.
+    // One macro step: form the closure inputs, run the network, take the large step (whose force
+    // pass also saves the sorted state it starts from into `slot`), sort and run the density pass
+    // of the new state (the judge's input, and the next step's), and copy the density maximum to
+    // the host.  Captured once per slot and replayed.
     auto enqueue_macro = [&](int slot) -> int {
-        WC_HIP(hipMemcpyAsync(c->cl_backup[slot][0], c->b.posrho[1], (size_t)n * sizeof(float4),
-                              hipMemcpyDeviceToDevice, c->stream));
-        WC_HIP(hipMemcpyAsync(c->cl_backup[slot][1], c->b.velp[1], (size_t)n * sizeof(float4),
-                              hipMemcpyDeviceToDevice, c->stream));
-        if (c->b.ids[0])
-            WC_HIP(hipMemcpyAsync(c->cl_backup_ids[slot], c->b.ids[1], (size_t)n * 4, hipMemcpyDeviceToDevice,
-                                  c->stream));
         if (net && c->neighbour_list) {
             WC_HIP(closure_infer_from_lists(c, net, (float)K, c->cl_y));
         } else if (net) {
@@ -1370,8 +1363,12 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
                                                c->stream));
             WC_HIP(wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream));
         }
+        wc::RecordBackup backup;
+        backup.posrho = c->cl_backup[slot][0];
+        backup.velp = c->cl_backup[slot][1];
+        backup.ids = c->b.ids[0] ? c->cl_backup_ids[slot] : nullptr;
         WC_HIP(wc::launch_closure_update(c->b, dt_large, net ? c->cl_y : nullptr, g, nullptr, c->cl_smooth,
-                                         c->cl_substeps ? (float)K : 0.0f, c->neighbour_list, c->stream));
+                                         c->cl_substeps ? (float)K : 0.0f, c->neighbour_list, c->stream, backup));
         int r = sort_density();
         if (r) return r;
         WC_HIP(hipMemcpyAsync(c->cl_host_rho + slot, c->b.rho_max, 4, hipMemcpyDeviceToHost, c->stream));

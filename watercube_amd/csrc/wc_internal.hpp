@@ -114,8 +114,16 @@ hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows,
 // neighbours when `smooth`; acc_out (may be NULL) receives the plain acceleration.  substeps = K
 // > 0 integrates as K steps of dt / K under the frozen force (0: one symplectic Euler step).
 // With `lists` the neighbours come from the last density pass's neighbour lists.
+// Where the force pass also copies each particle's sorted record (and identity) before
+// integrating it: the learned closure's saved state, written without a separate copy.
+struct RecordBackup {
+    float4* posrho = nullptr;
+    float4* velp = nullptr;
+    uint32_t* ids = nullptr;
+};
 hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
-                                 float* acc_out, bool smooth, float substeps, bool lists, hipStream_t stream);
+                                 float* acc_out, bool smooth, float substeps, bool lists, hipStream_t stream,
+                                 const RecordBackup& backup = RecordBackup{});
 
 // Tuning variant of the neighbour passes; all variants produce identical results.
 struct PassVariant {

@@ -727,7 +727,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                                                   float dt_arg, const float4* __restrict__ corr, float corr_scale,
                                                   float4* __restrict__ acc_out, const uint32_t* __restrict__ nlist,
                                                   const uint32_t* __restrict__ ncount, float substeps,
-                                                  const uint32_t* __restrict__ div_flags) {
+                                                  const uint32_t* __restrict__ div_flags, RecordBackup backup) {
     __shared__ uint32_t s_list[CAP * BLOCK];
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
@@ -738,6 +738,11 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
     const uint32_t t = d.p_off + lt;
     const float4 p = posrho_in[t];
     const float4 pv = velp_in[t];
+    if (backup.posrho) {
+        backup.posrho[t] = p;
+        backup.velp[t] = pv;
+        if (backup.ids && id_sorted) backup.ids[t] = id_sorted[t];
+    }
     const lds_ptr<uint32_t> list = (lds_ptr<uint32_t>)s_list + threadIdx.x;
     ForceSums f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     V3 csum{0.0f, 0.0f, 0.0f};  // MODE 2: weighted sum of neighbour corrections
@@ -1246,12 +1251,12 @@ hipError_t launch_update(const DeviceBuffers& b, int v, bool list, hipStream_t s
             hipLaunchKernelGGL((k_update<B, C, K, P, 0, true>), dim3(blocks), dim3(B), 0, s, b.sc,         \
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
                                b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, b.nlist,    \
-                               b.ncount, 0.0f, b.rho_max + 1);                                                  \
+                               b.ncount, 0.0f, b.rho_max + 1, RecordBackup{});                                  \
         else                                                                                              \
             hipLaunchKernelGGL((k_update<B, C, K, P, 0, false>), dim3(blocks), dim3(B), 0, s, b.sc,        \
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0],            \
                                b.scene_done, b.ids[1], b.ids[0], 0.0f, nullptr, 0.0f, nullptr, nullptr,    \
-                               nullptr, 0.0f, b.rho_max + 1);                                                   \
+                               nullptr, 0.0f, b.rho_max + 1, RecordBackup{});                                   \
         return hipGetLastError();                                                                         \
     }
     WC_VARIANTS(WC_LAUNCH)
@@ -1284,7 +1289,8 @@ hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t
 }
 
 hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* corr, float corr_scale,
-                                 float* acc_out, bool smooth, float substeps, bool lists, hipStream_t s) {
+                                 float* acc_out, bool smooth, float substeps, bool lists, hipStream_t s,
+                                 const RecordBackup& backup) {
     if (b.sc.blocks64 == 0) return hipSuccess;
     if (lists && (!b.nlist || !b.ncount)) return hipErrorInvalidValue;
     if (lists) {
@@ -1292,24 +1298,24 @@ hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* 
             hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, true>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done,
                                b.ids[1], b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps, b.rho_max + 1);
+                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps, b.rho_max + 1, backup);
         else
             hipLaunchKernelGGL((k_update<64, 32, 4, true, 1, true>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                                b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done,
                                b.ids[1], b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps, b.rho_max + 1);
+                               reinterpret_cast<float4*>(acc_out), b.nlist, b.ncount, substeps, b.rho_max + 1, backup);
         return hipGetLastError();
     }
     if (smooth)
         hipLaunchKernelGGL((k_update<64, 32, 4, true, 2, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1);
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1, backup);
     else
         hipLaunchKernelGGL((k_update<64, 32, 4, true, 1, false>), dim3(b.sc.blocks64), dim3(64), 0, s, b.sc,
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
-                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1);
+                           reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1, backup);
     return hipGetLastError();
 }
 
