@@ -23,7 +23,9 @@ import json
 import sys
 from pathlib import Path
 
-VALU_WAVE_INSTR_PER_S = 256 * 2.4e9
+SIMDS = 256 * 4
+CLOCK_HZ = 2.4e9
+VALU_WAVE_INSTR_PER_S = 256 * CLOCK_HZ
 STEP_KERNELS = ("k_hash", "k_scan", "k_place", "k_gather", "k_density", "k_update")
 
 
@@ -137,6 +139,20 @@ def _sq_table(ks, s, traffic, workload, particles, command, out, kernels) -> Non
         lines.append(f"| {k} | {c['SQ_WAVES']:.0f} | {c['SQ_INSTS_VALU']:.4g} | {c['SQ_INSTS_SALU']:.4g} | "
                      f"{c['SQ_INSTS_VMEM_RD']:.4g} | {c['SQ_INSTS_LDS']:.4g} | "
                      f"{c['SQ_INSTS_VALU'] / max(c['SQ_WAVES'], 1):.0f} | {util:.2f} |")
+    mfma = [k for k in kernels if k in s and k in ks and s[k].get("SQ_INSTS_MFMA", 0) > 0
+            and "SQ_VALU_MFMA_BUSY_CYCLES" in s[k]]
+    if mfma:
+        lines += ["", "Matrix cores: SQ_VALU_MFMA_BUSY_CYCLES is summed over the 1,024 SIMDs, so busy = it / "
+                  "(duration x 1,024 x 2.4 GHz).  Transcendental instructions are SQ_INSTS_VALU_TRANS_F32.", "",
+                  "| kernel | MFMA instructions | MFMA per wave | matrix cores busy | transcendental per wave |",
+                  "|---|---|---|---|---|"]
+        for k in mfma:
+            c = s[k]
+            busy = c["SQ_VALU_MFMA_BUSY_CYCLES"] / (ks[k]["avg_ns"] * 1e-9 * SIMDS * CLOCK_HZ)
+            trans = c.get("SQ_INSTS_VALU_TRANS_F32")
+            trans_s = f"{trans / max(c['SQ_WAVES'], 1):.0f}" if trans is not None else "-"
+            lines.append(f"| {k} | {c['SQ_INSTS_MFMA']:.4g} | {c['SQ_INSTS_MFMA'] / max(c['SQ_WAVES'], 1):.0f} | "
+                         f"{busy:.2f} | {trans_s} |")
     (out / f"sq_{workload}.md").write_text("\n".join(lines) + "\n")
 
 
