@@ -247,6 +247,28 @@ def test_rejecting_judge_reproduces_plain_small_steps(lists, oracle):
         assert b.particles(1).tobytes() == ref.buf1.tobytes()
 
 
+def test_rejecting_judge_restores_identities():
+    """Identities tracked through a learned run whose judge rejects every step: each rejected
+    step restores the state, identities included, that the force pass saved, so the run ends
+    with the particles and identities of plain small steps."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    K, macro = 4, 5
+    with Fluid(p, parts) as a:
+        a.track_ids(True)
+        a.step(dt, K * macro)
+        want, want_ids = a.particles(1), a.ids(1)
+    net = _zero_net()
+    with Fluid(p, parts) as b:
+        b.set_tuning(2, 2, 1)
+        b.track_ids(True)
+        st = L.wc_closure_stats()
+        L.check(L.lib().wc_closure_run(b.ctx, net.h, C.c_float(dt), K, macro, C.c_float(-1.0), C.byref(st)),
+                "wc_closure_run")
+        assert st.fallbacks == macro
+        assert b.particles(1).tobytes() == want.tobytes()
+        assert np.array_equal(b.ids(1), want_ids)
+
+
 def test_identities_follow_particles():
     p, parts, dt, _ = scenes.workload("dam_break_16k")
     with Fluid(p, parts) as f:
