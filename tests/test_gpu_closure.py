@@ -269,6 +269,26 @@ def test_rejecting_judge_restores_identities():
         assert np.array_equal(b.ids(1), want_ids)
 
 
+def test_closure_entry_points_reject_what_they_cannot_run():
+    """Closure inputs formed from the lists need the lists; the learned closure runs on
+    single-scene contexts only.  Both are refused with WC_ERR_INVALID_ARG, not run."""
+    from watercube_amd import batch
+
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    net = _zero_net()
+    y = DeviceArray((p.num_particles, 4), np.float32)
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, 0)
+        assert L.lib().wc_closure_infer(f.ctx, net.h, C.c_float(4.0), y.ptr) == L.WC_ERR_INVALID_ARG
+    with batch.SceneBatch(batch.scenes_256(seed=1, num_scenes=2, n_min=4096, n_max=8192)) as b:
+        x = DeviceArray((b.total_particles, 208), np.uint16)
+        assert L.lib().wc_closure_features(b.ctx, C.c_float(4.0), None, 0, x.ptr) == L.WC_ERR_INVALID_ARG
+        assert L.lib().wc_closure_infer(b.ctx, net.h, C.c_float(4.0), y.ptr) == L.WC_ERR_INVALID_ARG
+        st = L.wc_closure_stats()
+        assert L.lib().wc_closure_run(b.ctx, net.h, C.c_float(dt), 4, 1, C.c_float(0.01),
+                                      C.byref(st)) == L.WC_ERR_INVALID_ARG
+
+
 def test_identities_follow_particles():
     p, parts, dt, _ = scenes.workload("dam_break_16k")
     with Fluid(p, parts) as f:
