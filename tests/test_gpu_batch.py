@@ -45,6 +45,29 @@ def test_batch_matches_each_scene_alone(oracle):
                 assert lat[s] == 0
 
 
+def test_batch_of_ragged_tiny_scenes(oracle):
+    """Scenes of 1, 2, 63, 64, 65, 129, 257 and 4,097 particles: every pass maps its workgroups to
+    scenes by each scene's first block, and these sizes put scene boundaries inside and at the
+    edges of waves and workgroups.  Each scene must still match the CPU restatement run alone."""
+    parts_all, s = batch.fill("block", 5000, seed=21)
+    scenes = []
+    for i, n in enumerate((1, 2, 63, 64, 65, 129, 257, 4097)):
+        parts = np.ascontiguousarray(parts_all[(7 * i) % 200:(7 * i) % 200 + n])
+        p = batch.scene_params(len(parts), s)
+        scenes.append((p, parts, float(np.float32(2e-4 * s / 0.015)), f"n={n}"))
+    steps = 10
+    with batch.SceneBatch(scenes) as b:
+        b.step(steps)
+        b.synchronize()
+        for i, (p, parts, dt, what) in enumerate(scenes):
+            print(f"scene {i} {what}", flush=True)
+            st = oracle.OracleState(p, parts)
+            st.step(dt, steps)
+            assert b.scene_particles(i).tobytes() == st.buf1.tobytes(), f"scene {what} differs"
+            counts, offsets = b.scene_cells(i)
+            assert np.array_equal(counts, st.counts) and np.array_equal(offsets, st.offsets)
+
+
 def test_batch_tuning_keeps_results(oracle):
     scenes = _small_batch()
     with batch.SceneBatch(scenes) as b:
