@@ -227,6 +227,30 @@ def test_nan_and_infinite_inputs(lists, oracle):
         assert np.isnan(st.buf1["density"]).sum() > 200  # the NaN did spread
 
 
+def test_few_particles_on_a_fine_grid(oracle):
+    """3,000 particles on the 250^3 grid of the 17.8M-particle scene: 15.6M cells, nearly all
+    empty, so the scan and the cell ranges run over long empty stretches; two clusters and a
+    few loners.  Bit-exact against the oracle, in both neighbour-list modes."""
+    p = L.preset(L.WC_PRESET_BLOCK_SPHERE_17M)
+    rng = np.random.default_rng(17)
+    h = float(p.kernel_radius)
+    pos = np.concatenate([rng.uniform(0.2, 0.2 + 6 * h, (1500, 3)), rng.uniform(0.7, 0.7 + 8 * h, (1490, 3)),
+                          rng.uniform(0.0, 1.0, (10, 3))]).astype(np.float32)
+    parts = _records(pos)
+    p.num_particles = parts.size
+    _log(f"case fine grid: n={parts.size} R={p.grid_res}")
+    st = oracle.OracleState(p, parts)
+    st.step(2.5e-5, 4)
+    for lists in (0, 1):
+        with Fluid(p, parts) as f:
+            f.set_tuning(2, 2, lists)
+            f.step(2.5e-5, 4)
+            f.synchronize()
+            _check_records(f.particles(1), st.buf1, f"fine grid lists={lists}")
+            assert np.array_equal(f.sort_.getCountBuffer(), st.counts)
+            assert np.array_equal(f.sort_.getOffsetBuffer(), st.offsets)
+
+
 def test_dense_cluster_uses_fallback_path(oracle):
     """400 particles packed into a quarter-cell: every particle has ~399 neighbours, more than the
     neighbour list holds, so the force pass walks the cells again; results must not change."""
