@@ -276,6 +276,36 @@ def test_mouse_force_trajectory(oracle):
     _trajectory_case(p, parts, dt, 20, oracle, "mouse force")
 
 
+def test_rotating_gravity_and_mouse_ray_between_steps(oracle):
+    """Fluid::update sets gravity (rotated with the fluid, Fluid.cpp:259-263) and the mouse ray
+    before every step.  Changing them between captured graph steps must take effect at once, as
+    it does in the oracle when its parameters change between steps."""
+    from watercube_amd.fluid import gravity_direction
+
+    p, parts, dt, _ = scenes.workload("reference_default")
+    st = oracle.OracleState(p.copy(), parts)
+    quats = [(1.0, 0.0, 0.0, 0.0), (0.9238795, 0.3826834, 0.0, 0.0), (0.7071068, 0.0, 0.0, 0.7071068)]
+    with Fluid(p, parts) as f:
+        for i, q in enumerate(quats):
+            f.updateGravity(q, 9.81)
+            if i == 2:
+                f.params.mouse_force = 1
+                f.params.camera_position[:] = (0.3, 0.2, -2.0)
+                f.params.mouse_ray_direction[:] = (0.0, 0.05, 0.998749)
+                L.check(L.lib().wc_set_params(f.ctx, L.C.byref(f.params)), "wc_set_params")
+            d = gravity_direction(q)
+            for a in range(3):
+                st.params.gravity[a] = float(np.float32(d[a]) * np.float32(9.81))
+            if i == 2:
+                st.params.mouse_force = 1
+                st.params.camera_position[:] = (0.3, 0.2, -2.0)
+                st.params.mouse_ray_direction[:] = (0.0, 0.05, 0.998749)
+            f.step(dt, 4)
+            st.step(dt, 4)
+            f.synchronize()
+            _check_records(f.particles(1), st.buf1, f"after orientation {i}")
+
+
 def test_long_trajectory_dam_break(oracle):
     """1000 steps of the 16,384-particle dam break: bit-identical, so the relative error bound of
     1e-3 after 1000 steps holds with error 0."""
