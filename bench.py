@@ -19,7 +19,8 @@ Besides the throughput the line carries
   scale        the 17,777,216-particle scene;
   learned      plain and learned steps of K*dt against plain steps of dt (density errors,
                simulated time per second), and the closure network's matrix-core utilisation;
-  cpu_baseline the CPU restatement of the same step on this host's cores, on a bounded sample.
+  cpu_baseline the CPU restatement of the same step on this host's cores, on a bounded sample
+               (and, in single_thread, on one core).
 """
 from __future__ import annotations
 
@@ -98,6 +99,13 @@ def cpu_baseline(p, parts, dt, budget_s: float = 12.0, max_steps: int = 60) -> d
         elapsed += time.perf_counter() - t0
         done += 1
     n = int(p.num_particles)
+    # the same restatement on one core, over a few more steps of the same run
+    one_done, one_elapsed = 0, 0.0
+    while one_done < 5 and one_elapsed < budget_s / 2:
+        t0 = time.perf_counter()
+        st.step_timed(dt, 1, 1)
+        one_elapsed += time.perf_counter() - t0
+        one_done += 1
     return {
         "value": n * done / elapsed,
         "unit": "particle-steps/s",
@@ -105,6 +113,8 @@ def cpu_baseline(p, parts, dt, budget_s: float = 12.0, max_steps: int = 60) -> d
         "kind": "port",
         "sample": f"{done} steps of {WORKLOAD} ({n} particles, dt={dt}) from the same initial state, "
                   f"C restatement -O3 with OpenMP over particles for density/update, {elapsed:.1f} s",
+        "single_thread": {"value": n * one_done / one_elapsed, "unit": "particle-steps/s", "cores": 1,
+                          "sample": f"the next {one_done} steps on one core, {one_elapsed:.1f} s"},
     }
 
 
