@@ -1,6 +1,6 @@
 """Fixed workload for profiler runs (rocprofv3 kernel traces and PMC counters).
 
-    python3 -m watercube_amd.profile_workload [--workload block_1m] [--steps 20] [--mode graph|eager]
+    python3 -m watercube_amd.profile_workload [--workload block_1m|scenes_256] [--steps 20] [--mode graph|eager]
 
 Uploads the workload's initial state (the upload's record-split kernel `k_split` reads 32 bytes
 and writes 32 bytes per particle: a plain copy of known size, used to calibrate the byte
@@ -19,6 +19,38 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 
+def batch_workload(args) -> int:
+    """The 256-scene batch (log-uniform 4,096..262,144 particles, seed 0), stepped as the bench
+    steps it: one launch per pass over every scene, graph-captured.  Only kernel durations are
+    meaningful here: each scene's upload is its own record-split copy, of its own size."""
+    from watercube_amd import batch
+
+    scenes = batch.scenes_256(seed=0)
+    with batch.SceneBatch(scenes) as b:
+        print(f"profile_workload: scenes_256 n={b.total_particles} scenes={b.num_scenes}", flush=True)
+        if args.tuning == "auto":
+            t = b.tune(reps=3)
+            print(f"profile_workload: tuning {t['density_variant']},{t['update_variant']},{t['neighbour_list']}",
+                  flush=True)
+        else:
+            import ctypes as C
+
+            from watercube_amd import _lib as L
+
+            d, u, *rest = (int(x) for x in args.tuning.split(","))
+            t = L.wc_tuning()
+            L.check(L.lib().wc_get_tuning(b.ctx, C.byref(t)), "wc_get_tuning")
+            t.density_variant, t.update_variant, t.neighbour_list = d, u, (rest[0] if rest else 0)
+            L.check(L.lib().wc_set_tuning(b.ctx, C.byref(t)), "wc_set_tuning")
+        b.step(args.warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(args.steps)
+        b.synchronize()
+        print(f"profile_workload: {args.steps} graph steps in {(time.perf_counter() - t0) * 1e3:.2f} ms", flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="block_1m")
@@ -35,6 +67,8 @@ def main() -> int:
 
     from watercube_amd import Fluid, scenes
 
+    if args.workload == "scenes_256":
+        return batch_workload(args)
     p, parts, dt, _ = scenes.workload(args.workload)
     print(f"profile_workload: {args.workload} n={p.num_particles} copy-calibration bytes/dispatch "
           f"read={32 * p.num_particles} write={32 * p.num_particles}", flush=True)
