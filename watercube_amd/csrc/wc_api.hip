@@ -115,6 +115,7 @@ struct wc_ctx {
     std::vector<wc_params> params;   // per scene
     std::vector<SceneDesc> desc;     // per scene, mirrored in d_desc
     std::vector<uint32_t> tables;    // blk64 | blk128 | blk256 | tiles, each num_scenes + 1
+    std::vector<uint32_t> scene_tab;  // several scenes: scene of each 64/128/256-thread block and tile
     uint32_t num_scenes = 0;
     uint32_t total_particles = 0;
     uint32_t total_cells = 0;        // padded to scan tiles per scene
@@ -126,6 +127,7 @@ struct wc_ctx {
     DeviceBuffers b{};
     SceneDesc* d_desc = nullptr;
     uint32_t* d_tables = nullptr;
+    uint32_t* d_scene_tab = nullptr;
     wc_particle* staging = nullptr;  // particle_cap records for upload / download
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -190,6 +192,11 @@ int layout_scenes(wc_ctx* c) {
     }
     c->total_particles = (uint32_t)p_off;
     c->total_cells = (uint32_t)c_off;
+    c->scene_tab.clear();
+    if (S > 1) {
+        for (const uint32_t* first : {blk64, blk128, blk256, tiles})
+            for (uint32_t s = 0; s < S; ++s) c->scene_tab.insert(c->scene_tab.end(), first[s + 1] - first[s], s);
+    }
     return WC_OK;
 }
 
@@ -201,6 +208,13 @@ void fill_scene_views(wc_ctx* c) {
     v.blk128 = c->d_tables + (S + 1);
     v.blk256 = c->d_tables + 2 * (S + 1);
     v.tiles = c->d_tables + 3 * (S + 1);
+    v.scene64 = v.scene128 = v.scene256 = v.scene_tile = nullptr;
+    if (S > 1 && c->d_scene_tab) {
+        v.scene64 = c->d_scene_tab;
+        v.scene128 = v.scene64 + c->tables[S];
+        v.scene256 = v.scene128 + c->tables[2 * (size_t)(S + 1) - 1];
+        v.scene_tile = v.scene256 + c->tables[3 * (size_t)(S + 1) - 1];
+    }
     v.num_scenes = S;
     v.total_particles = c->total_particles;
     v.total_tiles = c->tables[4 * (size_t)(S + 1) - 1];
@@ -213,6 +227,9 @@ int upload_tables(wc_ctx* c) {
     WC_HIP(hipMemcpyAsync(c->d_desc, c->desc.data(), c->desc.size() * sizeof(SceneDesc), hipMemcpyHostToDevice,
                           c->stream));
     WC_HIP(hipMemcpyAsync(c->d_tables, c->tables.data(), c->tables.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!c->scene_tab.empty())
+        WC_HIP(hipMemcpyAsync(c->d_scene_tab, c->scene_tab.data(), c->scene_tab.size() * 4, hipMemcpyHostToDevice,
+                              c->stream));
     WC_HIP(hipStreamSynchronize(c->stream));
     fill_scene_views(c);
     return WC_OK;
@@ -222,10 +239,10 @@ void free_ctx_buffers(wc_ctx* c) {
     DeviceBuffers& b = c->b;
     void* ptrs[] = {b.posrho[0],   b.posrho[1],   b.velp[0],     b.velp[1],     b.key,       b.rank,
                     b.slot_tmp,    b.sorted,      b.skey,        b.counts,      b.range,     b.scan_status,
-                    b.scan_ticket, b.error_flag,  b.scene_done,  b.step_start,  c->staging,  c->d_desc,
+                    b.error_flag,  b.scene_done,  b.step_start,  c->staging,  c->d_desc,
                     c->d_tables,   b.rho_max,     b.ids[0],      b.ids[1],      c->cl_x,     c->cl_y,
                     c->cl_backup[0][0], c->cl_backup[0][1], c->cl_backup[1][0], c->cl_backup[1][1],
-                    c->cl_backup_ids[0], c->cl_backup_ids[1], c->cl_rows, b.nlist, b.ncount};
+                    c->cl_backup_ids[0], c->cl_backup_ids[1], c->cl_rows, b.nlist, b.ncount, c->d_scene_tab};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->cl_host_rho) (void)hipHostFree(c->cl_host_rho);
@@ -234,6 +251,7 @@ void free_ctx_buffers(wc_ctx* c) {
     c->staging = nullptr;
     c->d_desc = nullptr;
     c->d_tables = nullptr;
+    c->d_scene_tab = nullptr;
     c->cl_x = nullptr;
     c->cl_y = nullptr;
     c->cl_backup[0][0] = c->cl_backup[0][1] = c->cl_backup[1][0] = c->cl_backup[1][1] = nullptr;
@@ -349,7 +367,6 @@ int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
     alloc((void**)&b.counts, cells * 4 + 64);
     alloc((void**)&b.range, cells * 8 + 64);
     alloc((void**)&b.scan_status, tiles * 8 + 8);
-    alloc((void**)&b.scan_ticket, 64);
     alloc((void**)&b.error_flag, 64);
     alloc((void**)&b.scene_done, (size_t)num * 8 + 8);
     alloc((void**)&b.step_start, 64);
@@ -357,6 +374,7 @@ int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
     alloc((void**)&c->staging, n * sizeof(wc_particle));
     alloc((void**)&c->d_desc, (size_t)num * sizeof(SceneDesc));
     alloc((void**)&c->d_tables, c->tables.size() * 4);
+    if (!c->scene_tab.empty()) alloc((void**)&c->d_scene_tab, c->scene_tab.size() * 4);
     if (!ok) {
         destroy_ctx(c);
         return fail(WC_ERR_NO_MEMORY, "device allocation failed (%u particles, %u cells)", c->total_particles,

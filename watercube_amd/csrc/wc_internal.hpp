@@ -3,8 +3,8 @@
 // A context holds one or more independent scenes.  Their particles are concatenated in the
 // particle buffers and their cell grids in the cell arrays; a device-side table describes each
 // scene (constants, first particle slot, first cell, step size).  Kernels launch once per pass
-// over all scenes: a workgroup finds its scene by a binary search of the per-scene first-block
-// table, which is uniform across the workgroup and so runs on the scalar unit.
+// over all scenes: a workgroup finds its scene with one scalar load from a per-block scene table
+// (a context with one scene skips it).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,6 +71,12 @@ struct Scenes {
     const uint32_t* blk128;  // same for 128-thread workgroups
     const uint32_t* blk256;  // same for 256-thread workgroups
     const uint32_t* tiles;   // first scan tile of each scene; num_scenes + 1 entries
+    // multi-scene contexts: the scene of every 64-, 128-, 256-thread block and of every scan tile
+    // (NULL with one scene)
+    const uint32_t* scene64;
+    const uint32_t* scene128;
+    const uint32_t* scene256;
+    const uint32_t* scene_tile;
     uint32_t num_scenes;
     uint32_t total_particles;
     uint32_t total_tiles;
@@ -90,7 +96,6 @@ struct DeviceBuffers {
     uint32_t* counts;    // per cell: atomic histogram; the scan reads and zeroes it
     uint2* range;        // per cell: (first slot, count); slots are context-global
     unsigned long long* scan_status;  // one look-back word per scan tile
-    uint32_t* scan_ticket;            // dynamic tile counter
     uint32_t* error_flag;             // nonzero after a failed bounded wait
     unsigned long long* scene_done;   // per scene: wall clock when its force pass finished
     unsigned long long* step_start;   // wall clock when the last step's hash pass started

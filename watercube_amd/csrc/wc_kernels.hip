@@ -47,16 +47,13 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
     return x * q + min(x, r) + k;
 }
 
-// Scene owning logical block b: the largest s with first[s] <= b.  `first` has num_scenes + 1
-// entries; scenes without particles have first[s] == first[s+1] and are skipped.
-__device__ __forceinline__ uint32_t find_scene(const uint32_t* __restrict__ first, uint32_t num_scenes, uint32_t b) {
-    uint32_t lo = 0, hi = num_scenes;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+// Scene owning logical block (or tile) b: a table load when the context holds several scenes.
+__device__ __forceinline__ uint32_t scene_of(const uint32_t* __restrict__ table, uint32_t num_scenes, uint32_t b) {
+    return num_scenes == 1 ? 0u : table[b];
+}
+template <int BLOCK>
+__device__ __forceinline__ const uint32_t* scene_table(const Scenes& S) {
+    return BLOCK == 64 ? S.scene64 : (BLOCK == 128 ? S.scene128 : S.scene256);
 }
 
 template <int BLOCK>
@@ -113,7 +110,6 @@ __global__ __launch_bounds__(kHashBlock) void k_hash(Scenes S, const float4* __r
                                                      uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                      uint32_t* __restrict__ counts,
                                                      unsigned long long* __restrict__ scan_status,
-                                                     uint32_t* __restrict__ scan_ticket,
                                                      unsigned long long* __restrict__ scene_done,
                                                      unsigned long long* __restrict__ step_start,
                                                      uint32_t* __restrict__ rho_max) {
@@ -121,7 +117,6 @@ __global__ __launch_bounds__(kHashBlock) void k_hash(Scenes S, const float4* __r
     if (g < S.total_tiles) scan_status[g] = 0ull;
     if (g < S.num_scenes) scene_done[g] = 0ull;
     if (g == 0) {
-        *scan_ticket = 0u;
         *step_start = wall_clock64();
         rho_max[0] = 0u;
         rho_max[1] = kDivPending;  // the force pass's operands are unchecked until a density pass runs
@@ -130,7 +125,7 @@ __global__ __launch_bounds__(kHashBlock) void k_hash(Scenes S, const float4* __r
     bool valid = false;
     uint32_t k = kNoKey, i = 0, c_off = 0;
     if (b < S.blocks256) {
-        const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+        const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
         const SceneDesc& d = S.desc[s];
         const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
         valid = lt < d.c.n;
@@ -179,18 +174,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 __global__ __launch_bounds__(kScanThreads) void k_scan(Scenes S, uint32_t* __restrict__ counts,
                                                        uint2* __restrict__ range,
                                                        unsigned long long* __restrict__ status,
-                                                       uint32_t* __restrict__ ticket,
                                                        uint32_t* __restrict__ error_flag) {
-    __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
     const unsigned tid = threadIdx.x;
     const unsigned lane = lane_id();
     const unsigned wave = tid >> 6;
-    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;  // tiles are handed out in order: earlier tiles are running
-    const uint32_t s = find_scene(S.tiles, S.num_scenes, tile);
+    // Tile = workgroup index.  Each XCD dispatches its workgroups in index order, so the lowest
+    // unfinished tile has been dispatched, and it waits on nothing (its predecessors are done):
+    // the look-back always makes progress.  (A ticket counter gives the same guarantee, but
+    // every workgroup then queues on one atomic: about 25-30 ns per tile.)
+    const uint32_t tile = blockIdx.x;
+    const uint32_t s = scene_of(S.scene_tile, S.num_scenes, tile);
     const uint32_t first_tile = S.tiles[s];
     const SceneDesc& d = S.desc[s];
     const uint32_t ncells = d.c.ncells;
@@ -305,7 +300,7 @@ __global__ __launch_bounds__(kHashBlock) void k_place(Scenes S, const uint32_t* 
                                                       const uint2* __restrict__ range,
                                                       uint32_t* __restrict__ slot_tmp) {
     const uint32_t b = blockIdx.x;
-    const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+    const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
     const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
     if (lt >= d.c.n) return;
@@ -348,7 +343,7 @@ __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t*
                                                        const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id_out,
                                                        uint32_t* __restrict__ div_flags) {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+    const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
     const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
     if (lt >= d.c.n) return;
@@ -476,7 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(Scenes S, float4* __restrict_
     __shared__ Entry s_list[CAP * BLOCK];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(&rho_max[1], ~kDivPending);
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
+    const uint32_t s = scene_of(scene_table<BLOCK>(S), S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
     const StepConsts& c = d.c;
     const uint32_t lt = (b - first_blocks<BLOCK>(S)[s]) * BLOCK + threadIdx.x;
@@ -730,7 +725,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                                                   const uint32_t* __restrict__ div_flags, RecordBackup backup) {
     __shared__ uint32_t s_list[CAP * BLOCK];
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = find_scene(first_blocks<BLOCK>(S), S.num_scenes, b);
+    const uint32_t s = scene_of(scene_table<BLOCK>(S), S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
     const StepConsts& c = d.c;
     const uint32_t lt = (b - first_blocks<BLOCK>(S)[s]) * BLOCK + threadIdx.x;
@@ -883,7 +878,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
 __global__ __launch_bounds__(kHashBlock) void k_advect(Scenes S, float4* __restrict__ posrho,
                                                        float4* __restrict__ velp, float dt) {
     const uint32_t b = blockIdx.x;
-    const uint32_t s = find_scene(S.blk256, S.num_scenes, b);
+    const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
     const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
     if (lt >= d.c.n) return;
@@ -1164,16 +1159,16 @@ hipError_t launch_hash(const DeviceBuffers& b, hipStream_t s) {
     size_t blocks = S.blocks256;
     const size_t reset = blocks_for(S.total_tiles > S.num_scenes ? S.total_tiles : S.num_scenes, kHashBlock);
     if (reset > blocks) blocks = reset;
-    if (blocks == 0) blocks = 1;  // still resets the ticket and the step clock
+    if (blocks == 0) blocks = 1;  // still resets the step clock
     hipLaunchKernelGGL(k_hash, dim3((unsigned)blocks), dim3(kHashBlock), 0, s, S, b.posrho[0], b.key, b.rank,
-                       b.counts, b.scan_status, b.scan_ticket, b.scene_done, b.step_start, b.rho_max);
+                       b.counts, b.scan_status, b.scene_done, b.step_start, b.rho_max);
     return hipGetLastError();
 }
 
 hipError_t launch_scan(const DeviceBuffers& b, hipStream_t s) {
     if (b.sc.total_tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scan, dim3(b.sc.total_tiles), dim3(kScanThreads), 0, s, b.sc, b.counts, b.range,
-                       b.scan_status, b.scan_ticket, b.error_flag);
+                       b.scan_status, b.error_flag);
     return hipGetLastError();
 }
 
