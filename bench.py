@@ -40,26 +40,28 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = "block_1m"
 
 
-def algorithmic_bytes(n: int, ncells: int, list_entries: float = 0.0) -> dict:
-    """Bytes each kernel must move once per launch (every array touched once, neighbour
-    gathers counted once).  With neighbour lists (`list_entries` stored per particle on
-    average) the density pass also writes the lists and counts, and the force pass reads them
-    instead of the cell ranges."""
-    out = {
-        "hash": 16 * n + 8 * n + 4 * ncells,            # read pos+rho, write key+rank, histogram
-        "scan": 4 * ncells + 4 * ncells + 8 * ncells,   # read counts, zero counts, write (offset,count)
-        "place": 8 * n + 8 * ncells + 4 * n,            # read key+rank, offsets, write slot map
-        "gather": 4 * n + 4 * n + 8 * ncells + 32 * n + 32 * n + 8 * n,  # map, key, ranges, record in/out, order+key
-        "density": 16 * n + 4 * n + 8 * ncells + 8 * n,  # pos, cell key, ranges, write rho+p
-        "update": 32 * n + 4 * n + 8 * ncells + 32 * n,  # record in, cell key, ranges, record out
+def algorithmic_bytes(n: int, ncells: int) -> dict:
+    """Bytes each pass must move once per launch: every array touched once, each neighbour
+    gather counted once, nothing the implementation adds (keys kept beside the sorted records,
+    neighbour lists).  The reorder is one pass here (place + gather)."""
+    return {
+        "hash": 16 * n + 4 * n + 4 * ncells + 4 * ncells,  # read positions, write keys, histogram and its clearing
+        "scan": 8 * ncells,                                 # read counts, write offsets
+        "scatter": 36 * n + 4 * ncells + 32 * n,            # read key + record, offsets; write the record
+        "density": 16 * n + 8 * ncells + 8 * n,             # positions, cell ranges; write density + pressure
+        "update": 32 * n + 8 * ncells + 32 * n,             # records, cell ranges; write records
     }
-    if list_entries:
-        lists = int(round(4 * list_entries * n)) + 4 * n  # entries + counts
-        out["density"] += lists
-        out["update"] = 32 * n + lists + 32 * n          # record in, lists, record out
-    return out
 
 
+def overhead_bytes(n: int, list_entries: float) -> dict:
+    """Bytes the neighbour passes move beyond the algorithmic ones by this build's design: the
+    sorted cell key each pass reads, and with neighbour lists the list entries and counts the
+    density pass writes and the force pass reads back."""
+    lists = int(round(4 * list_entries * n)) + 4 * n if list_entries else 0
+    return {"density": 4 * n + lists, "update": 4 * n + lists}
+
+
+PROFILE_FROM_STEP = 520  # the committed counter profiles cover steps 521-570 of the workload
 VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9  # one wave64 VALU instruction per CU per cycle (no packed math)
 
 
@@ -284,7 +286,7 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=500, help="steps of the 256-scene batch (0 skips it)")
     ap.add_argument("--weights", default=str(ROOT / "watercube_amd" / "weights" / "closure.npy"),
                     help="closure network weights for the learned section (skipped if absent)")
-    ap.add_argument("--learned-horizon", type=int, default=100)
+    ap.add_argument("--learned-horizon", type=int, default=500)
     ap.add_argument("--scale-steps", type=int, default=50, help="steps of the 17.8M-particle scene (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -317,7 +319,12 @@ def main() -> int:
         elapsed = time.perf_counter() - t0
         print(f"# timed {args.steps} steps in {elapsed:.4f} s", file=sys.stderr, flush=True)
 
-        # per-kernel durations: HIP events on the context's stream around each launch
+        # per-kernel durations: HIP events on the context's stream around each launch, over the
+        # steps the committed counter profiles cover (steps 521-570), so both describe one state
+        done = args.warmup + args.steps
+        if done < PROFILE_FROM_STEP:
+            f.step(dt, PROFILE_FROM_STEP - done)
+            done = PROFILE_FROM_STEP
         prof = f.profile(dt, args.profile_steps)
         f.synchronize()
         state = f.particles(1)
@@ -328,22 +335,32 @@ def main() -> int:
     if not np.isfinite(state["position"]).all():
         print("bench: non-finite positions after the run", file=sys.stderr)
         return 1
+    prof_steps = f"{done + 1}-{done + args.profile_steps}"
 
     per_launch_ms = {k: v / args.profile_steps for k, v in prof.items()}
-    algo = algorithmic_bytes(n, ncells, list_entries)
-    dom = max(per_launch_ms, key=per_launch_ms.get)
-    achieved = algo[dom] / (per_launch_ms[dom] * 1e-3) / 1e9
-    traffic = read_traffic(dom)
     print("# per-launch ms: " + ", ".join(f"{k}={v:.4f}" for k, v in per_launch_ms.items()), file=sys.stderr,
           flush=True)
+    # the reorder is two kernels here (place, gather); its algorithmic bytes are counted once
+    pass_ms = {"hash": per_launch_ms["hash"], "scan": per_launch_ms["scan"],
+               "scatter": per_launch_ms["place"] + per_launch_ms["gather"],
+               "density": per_launch_ms["density"], "update": per_launch_ms["update"]}
+    algo = algorithmic_bytes(n, ncells)
+    extra = overhead_bytes(n, list_entries)
+    dom = max(per_launch_ms, key=per_launch_ms.get)
+    dom_s = per_launch_ms[dom] * 1e-3
+    achieved = algo[dom] / dom_s / 1e9
 
+    def measured(kernels):
+        vals = [read_traffic(k) for k in kernels]
+        return None if any(v is None for v in vals) else sum(vals)
+
+    pass_kernels = {"hash": ["hash"], "scan": ["scan"], "scatter": ["place", "gather"], "density": ["density"],
+                    "update": ["update"]}
+    traffic = measured([dom])
     valu = read_valu(dom)
-    valu_section = None if valu is None else {
-        "wave_instr_per_launch": valu, "achieved": valu / (per_launch_ms[dom] * 1e-3),
-        "peak": VALU_PEAK_WAVE_INSTR_PER_S, "unit": "wave-instr/s",
-        "frac": valu / (per_launch_ms[dom] * 1e-3) / VALU_PEAK_WAVE_INSTR_PER_S}
+    valu_frac = None if valu is None else valu / dom_s / VALU_PEAK_WAVE_INSTR_PER_S
     out = {
-        "metric": "particle-steps/s at 1M particles: plain vs learned large-dt, ≤1% density err",
+        "metric": "particle-steps/s at 1M particles: plain vs learned large-dt, \u22641% density err",
         "value": n * args.steps / elapsed,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -362,23 +379,35 @@ def main() -> int:
                               "neighbour_list": tuning["neighbour_list"]}},
         "simulated_s_per_s": dt * args.steps / elapsed,
         "pass_ms": per_launch_ms,
-        # per pass: algorithmic bytes and rocprofv3-measured HBM bytes (committed summary) over
-        # the live per-launch duration, in GB/s
+        "pass_ms_steps": prof_steps,
+        # per pass: algorithmic bytes and rocprofv3-measured HBM bytes (committed summary of the
+        # same steps) over the live per-launch duration, in GB/s
         "pass_gbs": {k: {"algorithmic": algo[k] / (v * 1e-3) / 1e9,
-                         "measured": None if read_traffic(k) is None else read_traffic(k) / (v * 1e-3) / 1e9}
-                     for k, v in per_launch_ms.items() if v > 0},
+                         "measured": None if measured(pass_kernels[k]) is None
+                         else measured(pass_kernels[k]) / (v * 1e-3) / 1e9}
+                     for k, v in pass_ms.items() if v > 0},
         "roofline": {
             "kernel": dom,
-            "bound": "hbm",
+            # the density pass is bound by its instruction issue, not by HBM: `frac` is its
+            # algorithmic bytes over the HBM peak, `valu_frac` its VALU issue rate over one wave64
+            # instruction per CU per cycle
+            "bound": "valu",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "hbm_frac": achieved / HBM_PEAK_GBS,
+            "valu_frac": valu_frac,
             "traffic": traffic,
+            "traffic_source": f"profiles/traffic_{WORKLOAD}.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                              f"over steps 521-570, calibrated on a 32 MB copy",
+            "duration_us": per_launch_ms[dom] * 1e3,
+            "duration_steps": prof_steps,
             "algorithmic_bytes": algo[dom],
+            "overhead_bytes": extra.get(dom),
             "neighbour_list_entries_per_particle": list_entries,
-            "binding": "valu",
-            "valu": valu_section,
+            "valu_wave_instr_per_launch": valu,
+            "valu_peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
         },
     }
     if args.batch_steps > 0:
@@ -389,14 +418,54 @@ def main() -> int:
         print(f"# scale: {out['scale']}", file=sys.stderr, flush=True)
     weights = Path(args.weights)
     if weights.exists():
-        out["learned"] = learned_section(p, parts, dt, weights, horizon=args.learned_horizon)
-        print(f"# learned: {json.dumps(out['learned'])}", file=sys.stderr, flush=True)
+        learned = learned_section(p, parts, dt, weights, horizon=args.learned_horizon)
+        # the whole comparison goes to stderr; the line keeps its summary
+        print(f"# learned: {json.dumps(learned)}", file=sys.stderr, flush=True)
+        out["learned"] = learned_summary(learned)
     else:
         print(f"# learned section skipped: no weights at {weights}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, parts, dt)
-    print(json.dumps(out), flush=True)
+    print(json.dumps(_rounded(out)), flush=True)
     return 0
+
+
+def _rounded(x, digits: int = 4):
+    """Floats to `digits` significant digits (the line stays short enough for a log tail); the
+    headline value and step time keep full precision."""
+    if isinstance(x, float):
+        return float(f"{x:.{digits}g}")
+    if isinstance(x, dict):
+        return {k: (v if k in ("value", "ms_per_step") and isinstance(v, float) else _rounded(v, digits))
+                for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_rounded(v, digits) for v in x]
+    return x
+
+
+def learned_summary(res: dict) -> dict:
+    """The learned comparison's headline figures: for each K and method the density errors,
+    fallbacks and speed-up over the plain dt run; the one-step error ratios; the network's
+    matrix-core use; the fastest method within 1 % density error."""
+    keep = ("mean_density_err", "p99_density_err", "max_density_err", "speedup_vs_plain", "fallbacks")
+    out = {"horizon_steps": res.get("horizon_steps"), "weights": res.get("weights"),
+           "best_within_1pct": res.get("best_within_1pct")}
+    for key, entry in res.items():
+        if not (key.startswith("K") and isinstance(entry, dict)):
+            continue
+        k = {}
+        for m, e in entry.items():
+            if m == "one_step":
+                k["one_step_ratio"] = e["ratio_to_plain_large"]
+            elif isinstance(e, dict):
+                k[m] = {f: e[f] for f in keep if f in e}
+                if "one_step_ratio" in e:
+                    k[m]["one_step_ratio"] = e["one_step_ratio"]
+        out[key] = k
+    for key in ("inference", "inference_from_lists"):
+        if key in res:
+            out[key] = {f: res[key][f] for f in ("ms", "mfma_utilisation") if f in res[key]}
+    return out
 
 
 if __name__ == "__main__":

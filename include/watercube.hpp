@@ -1,12 +1,12 @@
 // watercube.hpp -- C++ host classes over the C ABI of watercube.h, shaped like the reference's
 // src/core stage classes so that a caller of core::Fluid / core::Sort finds the same methods.
 //
-//   core::Fluid (src/core/Fluid.h:105-214)  ->  watercube::Fluid
+//   core::Fluid (src/core/Fluid.h:32-141)   ->  watercube::Fluid
 //   core::Sort  (src/core/Sort.h:20-71)     ->  watercube::Sort
 //
 // Differences from the reference, each deliberate:
 //   - the fluent setters return a reference to the object (the reference's thisRef() returns a
-//     copy, src/core/Fluid.h:158, so its chained setters configure a temporary);
+//     copy, src/core/Fluid.h:85, so its chained setters configure a temporary);
 //   - the GPU buffers live in HBM behind a wc_ctx, not in GL buffer objects; the getters return
 //     host copies;
 //   - failures throw watercube::Error carrying the C ABI's status code and message.
@@ -112,7 +112,7 @@ public:
         applyGravity();
         if (ctx_) check(wc_set_params(ctx_, &params_), "wc_set_params");
     }
-    // Mouse ray in the fluid's frame (Fluid::setMouseRay, src/core/Fluid.h:125).
+    // Mouse ray in the fluid's frame (Fluid::setMouseRay, src/core/Fluid.h:52).
     void setMouseRay(const float origin[3], const float direction[3]) {
         for (int a = 0; a < 3; ++a) {
             params_.camera_position[a] = origin[a];

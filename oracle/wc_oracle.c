@@ -391,7 +391,7 @@ WCO_API void wco_update(const wc_params* P, const wc_particle* in, const uint32_
 }
 
 /* advect.comp:20-59, in place: x = x + v dt, then per axis the clamp to [border, size - border]
- * with v *= -damping, border 0.01 and damping 0.3 fixed in the shader (advect.comp:33-34). */
+ * with v *= -damping, border 0.01 and damping 0.3 fixed in the shader (advect.comp:30-31). */
 WCO_API void wco_advect(const wc_params* P, wc_particle* ps, uint32_t n, float dt) {
     const float damping = 0.3f, border = 0.01f;
     const float hi = P->size - border;

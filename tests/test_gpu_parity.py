@@ -548,6 +548,67 @@ def test_force_pass_right_after_a_sort(lists, oracle):
         _check_records(f.particles(1), stm.buf1, "update after sort without density")
 
 
+def _oracle_after_upload(oracle, p, st_prev, state):
+    """The oracle's view of an upload: both buffers hold the new records (Fluid.cpp:146, 155),
+    counts and offsets are still those of the last sort."""
+    st = oracle.OracleState(p, state)
+    st.counts[:] = st_prev.counts
+    st.offsets[:] = st_prev.offsets
+    return st
+
+
+@pytest.mark.parametrize("lists", [0, 1])
+def test_update_right_after_an_upload(lists, oracle):
+    """runUpdateProg straight after new records were uploaded, with no sort or density pass in
+    between: the pass must find each particle's cell from its new position, as update.comp:141
+    does, with the last sort's cell ranges, and must neither read the neighbour lists of the
+    last density pass (they describe the old positions) nor trust its division-operand check.
+    Every particle moves by its own random offset, so cells and neighbour sets change."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    rng = np.random.default_rng(11)
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 3)
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, lists)
+        f.step(dt, 3)
+        moved = f.particles(1)
+        _check_records(moved, st.buf1, "before the upload")
+        moved["position"] += rng.uniform(-0.6, 0.6, moved["position"].shape).astype(np.float32) * np.float32(
+            p.bin_size)
+        moved["velocity"] += rng.uniform(-0.5, 0.5, moved["velocity"].shape).astype(np.float32)
+        _log(f"update after an upload, lists {lists}")
+        ref = _oracle_after_upload(oracle, p, st, moved)
+        f.upload(moved)
+        f.runUpdateProg(dt)
+        ref.update(dt)
+        _check_records(f.particles(1), ref.buf1, f"update after an upload, lists {lists}")
+
+
+@pytest.mark.parametrize("lists", [0, 1])
+def test_density_and_update_after_an_upload_with_a_tiny_velocity(lists, oracle):
+    """An upload whose records carry a velocity of 1e-35, then runDensityProg and runUpdateProg
+    without a sort: the density pass checks positions, densities and pressures but not
+    velocities, so the upload itself must leave the force pass dividing as IEEE does."""
+    p, parts, dt, _ = scenes.workload("dam_break_16k")
+    st = oracle.OracleState(p, parts)
+    st.step(dt, 2)
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, lists)
+        f.step(dt, 2)
+        state = f.particles(1)
+        state["velocity"][77, 0] = np.float32(1e-35)
+        state["velocity"][78, 2] = np.float32(-3e-38)
+        _log(f"density + update after an upload with a tiny velocity, lists {lists}")
+        ref = _oracle_after_upload(oracle, p, st, state)
+        f.upload(state)
+        f.runDensityProg()
+        ref.density()
+        _check_records(f.particles(2), ref.buf2, f"density after the upload, lists {lists}")
+        f.runUpdateProg(dt)
+        ref.update(dt)
+        _check_records(f.particles(1), ref.buf1, f"update after the upload, lists {lists}")
+
+
 PARAM_CASES = {
     # densities near 1e-13, below the 2^-40 operand range: the force pass divides as IEEE does
     "tiny mass": dict(particle_mass=1e-16),

@@ -402,7 +402,7 @@ def test_planted_mistake_is_caught(oracle):
 def test_cutoff_by_squared_distance_is_exact():
     """The kernels accept a neighbour when d2 < cut2, with cut2 the smallest float whose
     correctly rounded sqrt is >= h; the reference tests length(r) >= h (density.comp:117,
-    update.comp:169).  sqrtf is monotone, so the two agree for every float d2: checked on
+    update.comp:173).  sqrtf is monotone, so the two agree for every float d2: checked on
     every float within 4096 ulp of cut2 for each preset's h."""
     f32 = np.float32
     for h in (0.03, 0.016, 0.004, 0.04, 0.0123):

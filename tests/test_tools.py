@@ -61,3 +61,23 @@ def test_profile_summary_calibration(tmp_path):
         pytest.approx(2 * copy)
     table = (out / "kernels_synthetic.md").read_text()
     assert "| k_density | 2 | 0.25 |" in table  # two dispatches, 250 ns = 0.25 us
+
+
+def test_algorithmic_bytes_are_the_per_pass_minimum():
+    """Each array touched once per pass, each neighbour gather counted once: 176 N + 36 C bytes per
+    step over the five passes; 24 N + 8 C for the density pass (positions, cell ranges, density
+    and pressure out).  At the 1M block (C = 62^3) the density pass's figure is 25.9 MB, and a
+    282 us launch is 0.0115 of the 8 TB/s peak."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    n, c = 1_000_000, 62 ** 3
+    a = bench.algorithmic_bytes(n, c)
+    assert sum(a.values()) == 176 * n + 36 * c
+    assert a["density"] == 24 * n + 8 * c
+    assert a["update"] == 64 * n + 8 * c
+    assert a["density"] / 282e-6 / 1e9 / bench.HBM_PEAK_GBS == pytest.approx(0.0115, abs=5e-4)
+    o = bench.overhead_bytes(n, 29.5)
+    assert o["density"] == 4 * n + 118 * n + 4 * n
+    r = bench._rounded({"value": 1.23456789, "x": {"y": 1.23456789}, "z": [2.0 / 3.0]})
+    assert r == {"value": 1.23456789, "x": {"y": 1.235}, "z": [0.6667]}

@@ -477,6 +477,8 @@ int copy_records_in(wc_ctx* c, uint32_t first, const wc_particle* host, uint32_t
     for (int b = 0; b < 2; ++b)
         WC_HIP(wc::launch_split_records(c->staging + first, c->b.posrho[b] + first, c->b.velp[b] + first, n,
                                         c->stream));
+    // buffer 2 changed under the last sort: its cell keys, neighbour lists and division check
+    WC_HIP(wc::launch_invalidate(c->b, c->stream));
     WC_HIP(hipStreamSynchronize(c->stream));
     return WC_OK;
 }
@@ -651,7 +653,12 @@ WC_API int wc_set_params(wc_ctx* c, const wc_params* p) {
     if (rc) return rc;
     c->desc[0].dt = dt;
     drop_graph(c);
-    return upload_tables(c);
+    rc = upload_tables(c);
+    if (rc) return rc;
+    // keys, lists and the division check of buffer 2 were formed with the old constants
+    WC_HIP(wc::launch_invalidate(c->b, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    return WC_OK;
 }
 
 WC_API int wc_get_params(const wc_ctx* c, wc_params* out) {

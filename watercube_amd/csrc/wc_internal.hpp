@@ -46,9 +46,9 @@ struct StepConsts {
     float rho0, p0, k;   // rest density / pressure, stiffness
     float g[3];          // gravity vector
     float size;
-    float size_minus_h;  // size - kernelRadius (density.comp:62, update.comp:215)
+    float size_minus_h;  // size - kernelRadius (density.comp:62, update.comp:78)
     float border;
-    float hi;            // size - border (update.comp:345)
+    float hi;            // size - border (update.comp:208)
     float damping;
     float max_speed;
     int32_t eos;
@@ -153,6 +153,9 @@ hipError_t launch_density(const DeviceBuffers& b, int variant, bool list, hipStr
 hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t* sel, hipStream_t stream);
 hipError_t launch_update(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
 hipError_t launch_advect(const DeviceBuffers& b, float dt, hipStream_t stream);
+// After the host changed particle buffer 2 or the parameters: recompute buffer 2's cell keys,
+// mark the neighbour lists overflowed and the force pass's division operands unchecked.
+hipError_t launch_invalidate(const DeviceBuffers& b, hipStream_t stream);
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n,
                                 hipStream_t stream);
 hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,

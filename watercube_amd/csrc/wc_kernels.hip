@@ -873,7 +873,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
 // ------------------------------------------------------------------------------------------
 // advect.comp:20-59 (the reference compiles it but Fluid::update never dispatches it): a
 // position-only step on buffer 1, x += v dt, then the cube clamp with the shader's own border
-// 0.01 and damping 0.3 (advect.comp:33-34), in place.
+// 0.01 and damping 0.3 (advect.comp:30-31), in place.
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kHashBlock) void k_advect(Scenes S, float4* __restrict__ posrho,
                                                        float4* __restrict__ velp, float dt) {
@@ -1131,6 +1131,30 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
 }
 
 // ------------------------------------------------------------------------------------------
+// After the host rewrites the particle buffers or the parameters: the cell key of every slot of
+// buffer 2 is recomputed from its position with the current constants (the neighbour passes
+// read it where the reference recomputes clamp(ivec3(p / binSize)), density.comp:88,
+// update.comp:141), the neighbour lists are marked overflowed (so every reader searches), and
+// the force pass's division operands are marked unchecked.  The cell ranges stay those of the
+// last sort, as the reference's counts and offsets do.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kHashBlock) void k_invalidate(Scenes S, const float4* __restrict__ posrho,
+                                                           uint32_t* __restrict__ skey, uint32_t* __restrict__ ncount,
+                                                           uint32_t* __restrict__ div_flags) {
+    const uint32_t b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) atomicOr(div_flags, kDivPending | kDivUnsafe);
+    if (b >= S.blocks256) return;
+    const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
+    const SceneDesc& d = S.desc[s];
+    const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
+    if (lt >= d.c.n) return;
+    const uint32_t t = d.p_off + lt;
+    const float4 p = posrho[t];
+    skey[t] = cell_key(d.c, p.x, p.y, p.z);
+    if (ncount) ncount[t] = kListOverflow;
+}
+
+// ------------------------------------------------------------------------------------------
 // Record <-> split-half conversion.
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kHashBlock) void k_split(const float4* __restrict__ rec, float4* __restrict__ posrho,
@@ -1311,6 +1335,13 @@ hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* 
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
                            reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1, backup);
+    return hipGetLastError();
+}
+
+hipError_t launch_invalidate(const DeviceBuffers& b, hipStream_t s) {
+    const unsigned blocks = b.sc.blocks256 ? b.sc.blocks256 : 1;  // block 0 always marks the flags
+    hipLaunchKernelGGL(k_invalidate, dim3(blocks), dim3(kHashBlock), 0, s, b.sc, b.posrho[1], b.skey, b.ncount,
+                       b.rho_max + 1);
     return hipGetLastError();
 }
 

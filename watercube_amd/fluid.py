@@ -65,7 +65,7 @@ class Sort:
 
 
 class Fluid:
-    """``core::Fluid`` (src/core/Fluid.h:105-214), simulation half, on one GPU.
+    """``core::Fluid`` (src/core/Fluid.h:32-141), simulation half, on one GPU.
 
     ``setup()`` allocates the context and uploads the initial particles (src/core/Fluid.cpp:203-235);
     ``update(time)`` is one ``Fluid::update`` (src/core/Fluid.cpp:342-354) with
