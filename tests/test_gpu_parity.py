@@ -268,6 +268,74 @@ def test_dense_cluster_uses_fallback_path(oracle):
         _check_records(f.particles(1), st.buf1, "dense cluster, neighbour lists")
 
 
+def _cell_box(p, cell, margin=0.05):
+    """The inside of grid cell `cell` = (x, y, z), shrunk by `margin` of the cell size."""
+    b = float(p.bin_size)
+    lo = np.array(cell, np.float64) * b + margin * b
+    return lo, lo + (1 - 2 * margin) * b
+
+
+def _crowded_sort_check(p, parts, oracle, what):
+    st, keys, counts, offsets, order, buf2 = _run_sort_pass(p, parts, oracle)
+    n = p.num_particles
+    assert np.array_equal(keys, st.keys[:n]), f"{what}: keys"
+    assert np.array_equal(counts, st.counts) and np.array_equal(offsets, st.offsets), f"{what}: cells"
+    assert np.array_equal(order, np.argsort(keys, kind="stable").astype(np.uint32)), f"{what}: not stable"
+    assert np.array_equal(order, st.sorted[:n]), f"{what}: order"
+    _check_records_nan_aware(buf2, st.buf2, f"{what}: sorted records")
+    return counts
+
+
+@pytest.mark.parametrize("lists", [0, 1])
+def test_crowded_cells_keep_input_order(lists, oracle):
+    """Cells of 3,000, 700 and 257 particles -- above the 256 the gather ranks directly, so their
+    input indices are sorted by the crowded-cell path -- among 5,000 scattered particles, in
+    shuffled input order: the sorted order must be numpy's stable argsort of the keys, and two
+    whole steps (every crowded particle overflows its neighbour list) must match the oracle."""
+    p = _params_for(0)
+    rng = np.random.default_rng(31)
+    pos = [rng.uniform(0.05, 0.95, (5000, 3))]
+    for cell, m in (((16, 16, 16), 3000), ((4, 20, 9), 700), ((30, 2, 17), 257)):
+        lo, hi = _cell_box(p, cell)
+        pos.append(rng.uniform(lo, hi, (m, 3)))
+    pos = np.concatenate(pos).astype(np.float32)
+    parts = _records(pos[rng.permutation(len(pos))])
+    p.num_particles = parts.size
+    _log(f"crowded cells: n={parts.size}, lists {lists}")
+    counts = _crowded_sort_check(p, parts, oracle, "crowded cells")
+    assert sorted(counts)[-3:] == [257, 700, 3000]
+    st = oracle.OracleState(p, parts)
+    st.step(2e-5, 2)
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, lists)
+        f.step(2e-5, 2)
+        _check_records(f.particles(1), st.buf1, f"crowded cells, two steps, lists {lists}")
+
+
+def test_very_crowded_cells_and_nan_positions(oracle):
+    """20,000 particles in one cell (sorted in 4,096-entry chunks, then three merge passes) and
+    12,000 NaN positions, which all hash to cell 0, among 3,000 others, in shuffled order: the sort
+    pass bit for bit, then one whole step (NaNs in the same places, every other value the same
+    bits)."""
+    p = _params_for(0)
+    rng = np.random.default_rng(47)
+    lo, hi = _cell_box(p, (12, 25, 7))
+    pos = np.concatenate([rng.uniform(lo, hi, (20000, 3)), rng.uniform(0.05, 0.95, (3000, 3)),
+                          np.full((12000, 3), np.nan)]).astype(np.float32)
+    parts = _records(pos[rng.permutation(len(pos))])
+    p.num_particles = parts.size
+    _log(f"very crowded cells: n={parts.size}")
+    counts = _crowded_sort_check(p, parts, oracle, "very crowded cells")
+    assert counts.max() == 20000 and counts[0] >= 12000
+    st = oracle.OracleState(p, parts)
+    st.step(2e-6, 1)
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, 1)
+        f.step(2e-6, 1)
+        f.synchronize()
+        _check_records_nan_aware(f.particles(1), st.buf1, "very crowded cells, one step")
+
+
 def test_mouse_force_trajectory(oracle):
     p, parts, dt, _ = scenes.workload("reference_default")
     p.mouse_force = 1

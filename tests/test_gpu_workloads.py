@@ -177,18 +177,23 @@ def _bf16_bits(a):
 
 
 def test_closure_1m_inputs_match_numpy_on_sampled_rows(oracle):
-    """The network's inputs at 1M (after 40 steps, neighbour lists on): 32 nearest accepted
+    """The network's inputs at 1M (after 300 steps, neighbour lists on): 32 nearest accepted
     neighbours by squared distance (ties: earlier in NEIGHBORHOOD order), listed in enumeration
-    order, relative positions / h and velocities, then the three scalars -- for 300 sampled rows
-    against a numpy restatement."""
+    order, relative positions / h and velocities, then the three scalars -- against a numpy
+    restatement for 150 random rows and the 150 rows with the most neighbours (whose lists hold
+    more than the 32 kept)."""
     p, parts, dt, _ = scenes.workload("block_1m")
     K = 4.0
-    rows = np.sort(np.random.default_rng(4).choice(p.num_particles, 300, replace=False)).astype(np.uint32)
     with Fluid(p, parts) as f:
         f.set_tuning(4, 0, 1)
-        f.step(dt, 40)
+        f.step(dt, 300)
         f.sort_.run()
         f.runDensityProg()
+        listed = f.neighbour_counts().astype(np.int64)
+        listed[listed == 0xFFFFFFFF] = 1 << 20
+        rng = np.random.default_rng(4)
+        rows = np.unique(np.concatenate([rng.choice(p.num_particles, 150, replace=False),
+                                         np.argsort(listed, kind="stable")[-150:]])).astype(np.uint32)
         s = f.particles(2)
         counts, offsets = f.sort_.getCountBuffer(), f.sort_.getOffsetBuffer()
         xd = DeviceArray((len(rows), 208), np.uint16)

@@ -242,7 +242,8 @@ void free_ctx_buffers(wc_ctx* c) {
                     b.error_flag,  b.scene_done,  b.step_start,  c->staging,  c->d_desc,
                     c->d_tables,   b.rho_max,     b.ids[0],      b.ids[1],      c->cl_x,     c->cl_y,
                     c->cl_backup[0][0], c->cl_backup[0][1], c->cl_backup[1][0], c->cl_backup[1][1],
-                    c->cl_backup_ids[0], c->cl_backup_ids[1], c->cl_rows, b.nlist, b.ncount, c->d_scene_tab};
+                    c->cl_backup_ids[0], c->cl_backup_ids[1], c->cl_rows, b.nlist, b.ncount, c->d_scene_tab,
+                    b.crowd,      b.crowd_count};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->cl_host_rho) (void)hipHostFree(c->cl_host_rho);
@@ -371,6 +372,9 @@ int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
     alloc((void**)&b.scene_done, (size_t)num * 8 + 8);
     alloc((void**)&b.step_start, 64);
     alloc((void**)&b.rho_max, 64);
+    b.crowd_cap = wc::crowd_capacity(n);
+    alloc((void**)&b.crowd, (size_t)b.crowd_cap * sizeof(uint2));
+    alloc((void**)&b.crowd_count, 64);
     alloc((void**)&c->staging, n * sizeof(wc_particle));
     alloc((void**)&c->d_desc, (size_t)num * sizeof(SceneDesc));
     alloc((void**)&c->d_tables, c->tables.size() * 4);
@@ -391,6 +395,7 @@ int create_context(const wc_params* scenes, uint32_t num, wc_ctx** out) {
     zero(b.scene_done, (size_t)num * 8 + 8);
     zero(b.step_start, 64);
     zero(b.rho_max, 64);
+    zero(b.crowd_count, 64);
     // rho_max[1] = 2: no density pass has checked the force pass's division operands yet
     if (e == hipSuccess) e = hipMemsetAsync(reinterpret_cast<char*>(b.rho_max) + 4, 2, 1, c->stream);
     zero(b.sorted, n * 4);

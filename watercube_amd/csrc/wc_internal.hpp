@@ -29,6 +29,13 @@ constexpr size_t kRecordPad = 8;
 // (the force pass then searches for that particle itself).
 constexpr uint32_t kNeighbourListMax = 64;
 constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
+// The reorder ranks a particle inside its cell by counting the cell's particles with a smaller
+// input index: at most kGatherDirectMax of them.  More crowded cells (a dense cluster, or every
+// NaN position, which hashes to cell 0) go to a work list, and one workgroup per listed cell sorts
+// its input indices instead (k_gather_crowded).  At most capacity / (kGatherDirectMax + 1) cells
+// can be listed.
+constexpr uint32_t kGatherDirectMax = 256;
+inline uint32_t crowd_capacity(uint64_t particles) { return (uint32_t)(particles / (kGatherDirectMax + 1) + 1); }
 
 // Per-scene constants, precomputed on the host in float so the device sees exactly the bits the
 // CPU transcription uses.
@@ -104,6 +111,9 @@ struct DeviceBuffers {
     uint32_t* ids[2];                 // particle identity per slot of buffer 1 / 2 (NULL: not tracked)
     uint32_t* nlist;                  // neighbour lists (NULL until the list mode is first used)
     uint32_t* ncount;
+    uint2* crowd;                     // cells with more than kGatherDirectMax particles: (cell, scene)
+    uint32_t* crowd_count;            // entries of `crowd` (reset by every sort)
+    uint32_t crowd_cap;
     Scenes sc;
 };
 

@@ -112,11 +112,13 @@ __global__ __launch_bounds__(kHashBlock) void k_hash(Scenes S, const float4* __r
                                                      unsigned long long* __restrict__ scan_status,
                                                      unsigned long long* __restrict__ scene_done,
                                                      unsigned long long* __restrict__ step_start,
-                                                     uint32_t* __restrict__ rho_max) {
+                                                     uint32_t* __restrict__ rho_max,
+                                                     uint32_t* __restrict__ crowd_count) {
     const uint32_t g = blockIdx.x * kHashBlock + threadIdx.x;
     if (g < S.total_tiles) scan_status[g] = 0ull;
     if (g < S.num_scenes) scene_done[g] = 0ull;
     if (g == 0) {
+        *crowd_count = 0u;
         *step_start = wall_clock64();
         rho_max[0] = 0u;
         rho_max[1] = kDivPending;  // the force pass's operands are unchecked until a density pass runs
@@ -332,7 +334,26 @@ __device__ __forceinline__ bool div_operand(float x) {
 // 4. stable rank + gather: the particle that arrived at slot t moves to the cell's first slot +
 //    (number of particles of its cell with a smaller input index), so every cell keeps input
 //    order; the record is copied there (reorder.comp:44) and the cell index kept beside it.
+//    Counting costs one pass over the cell per particle, so only cells of at most
+//    kGatherDirectMax particles are ranked this way; the first slot of a more crowded cell lists
+//    it for k_gather_crowded, which sorts the cell's input indices instead.
 // ------------------------------------------------------------------------------------------
+// Moves the particle with input index i to slot dst of the sorted buffer (reorder.comp:44).
+__device__ __forceinline__ void move_record(uint32_t i, uint32_t dst, uint32_t k, const float4* __restrict__ posrho_in,
+                                            const float4* __restrict__ velp_in, float4* __restrict__ posrho_out,
+                                            float4* __restrict__ velp_out, uint32_t* __restrict__ sorted,
+                                            uint32_t* __restrict__ skey, const uint32_t* __restrict__ id_in,
+                                            uint32_t* __restrict__ id_out, uint32_t* __restrict__ div_flags) {
+    sorted[dst] = i;
+    skey[dst] = k;
+    const float4 v = velp_in[i];
+    posrho_out[dst] = posrho_in[i];
+    velp_out[dst] = v;
+    if (id_out) id_out[dst] = id_in[i];
+    // velocities are operands of the force pass's quotients (the density pass checks the rest)
+    if (!(div_operand(v.x) && div_operand(v.y) && div_operand(v.z))) atomicOr(div_flags, kDivUnsafe);
+}
+
 __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t* __restrict__ slot_tmp,
                                                        const uint32_t* __restrict__ key,
                                                        const uint2* __restrict__ range,
@@ -341,7 +362,9 @@ __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t*
                                                        float4* __restrict__ posrho_out, float4* __restrict__ velp_out,
                                                        uint32_t* __restrict__ sorted, uint32_t* __restrict__ skey,
                                                        const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id_out,
-                                                       uint32_t* __restrict__ div_flags) {
+                                                       uint32_t* __restrict__ div_flags, uint2* __restrict__ crowd,
+                                                       uint32_t* __restrict__ crowd_count, uint32_t crowd_cap,
+                                                       uint32_t* __restrict__ error_flag) {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
@@ -353,19 +376,108 @@ __global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t*
     if (i - d.p_off >= d.c.n) return;  // only after a failed scan (error flag set)
     const uint32_t k = key[i];
     const uint2 rg = range[d.c_off + k];
+    if (rg.y > kGatherDirectMax) {
+        if (t == rg.x) {  // one entry per crowded cell
+            const uint32_t at = atomicAdd(crowd_count, 1u);
+            if (at < crowd_cap)
+                crowd[at] = make_uint2(d.c_off + k, s);
+            else
+                atomicOr(error_flag, 2u);  // cannot happen when the scan succeeded
+        }
+        return;
+    }
     const uint32_t end = min(rg.x + rg.y, lim);
     uint32_t r = 0;
     for (uint32_t u = rg.x; u < end; ++u) r += (slot_tmp[u] < i) ? 1u : 0u;
     const uint32_t dst = rg.x + r;
     if (dst >= lim) return;
-    sorted[dst] = i;
-    skey[dst] = k;
-    const float4 v = velp_in[i];
-    posrho_out[dst] = posrho_in[i];
-    velp_out[dst] = v;
-    if (id_out) id_out[dst] = id_in[i];
-    // velocities are operands of the force pass's quotients (the density pass checks the rest)
-    if (!(div_operand(v.x) && div_operand(v.y) && div_operand(v.z))) atomicOr(div_flags, kDivUnsafe);
+    move_record(i, dst, k, posrho_in, velp_in, posrho_out, velp_out, sorted, skey, id_in, id_out, div_flags);
+}
+
+// Crowded cells: one workgroup per listed cell (a grid-stride loop over the list, whose length the
+// gather pass left in crowd_count) sorts the cell's input indices, slot_tmp[first .. first + c),
+// in ascending order: chunks of kCrowdChunk by a bitonic sort in LDS, then merge passes between
+// slot_tmp and sorted over the same slots (each element's place in the merged run is its index in
+// its own run plus the number of smaller elements of the other run, by binary search; indices are
+// unique).  The particle with the r-th smallest index moves to slot first + r, exactly the rank
+// k_gather would give it.  Every loop is bounded by the cell's size: O(c log^2 c) work per cell.
+constexpr uint32_t kCrowdChunk = 4096;
+constexpr unsigned kCrowdBlocks = 32;
+
+__device__ __forceinline__ uint32_t count_below(const uint32_t* __restrict__ run, uint32_t n, uint32_t x) {
+    uint32_t a = 0, b = n;
+    while (a < b) {
+        const uint32_t m = (a + b) >> 1;
+        if (run[m] < x)
+            a = m + 1;
+        else
+            b = m;
+    }
+    return a;
+}
+
+__global__ __launch_bounds__(kHashBlock) void k_gather_crowded(
+    Scenes S, const uint2* __restrict__ crowd, const uint32_t* __restrict__ crowd_count, uint32_t crowd_cap,
+    uint32_t* __restrict__ slot_tmp, const uint2* __restrict__ range, const float4* __restrict__ posrho_in,
+    const float4* __restrict__ velp_in, float4* __restrict__ posrho_out, float4* __restrict__ velp_out,
+    uint32_t* __restrict__ sorted, uint32_t* __restrict__ skey, const uint32_t* __restrict__ id_in,
+    uint32_t* __restrict__ id_out, uint32_t* __restrict__ div_flags) {
+    __shared__ uint32_t s_v[kCrowdChunk];
+    const uint32_t listed = min(*crowd_count, crowd_cap);
+    for (uint32_t item = blockIdx.x; item < listed; item += gridDim.x) {
+        const uint2 e = crowd[item];  // (context cell, scene)
+        const SceneDesc& d = S.desc[e.y];
+        const uint2 rg = range[e.x];
+        const uint32_t first = rg.x, lim = d.p_off + d.c.n;
+        if (first < d.p_off || first >= lim) continue;  // only after a failed scan
+        const uint32_t c = min(rg.y, lim - first);
+        uint32_t* src = slot_tmp + first;
+        uint32_t* dst = sorted + first;
+        for (uint32_t lo = 0; lo < c; lo += kCrowdChunk) {
+            const uint32_t m = min(c - lo, kCrowdChunk);
+            uint32_t P = 1;
+            while (P < m) P <<= 1;
+            for (uint32_t j = threadIdx.x; j < P; j += kHashBlock) s_v[j] = j < m ? src[lo + j] : 0xFFFFFFFFu;
+            __syncthreads();
+            for (uint32_t k = 2; k <= P; k <<= 1) {
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t a = threadIdx.x; a < P; a += kHashBlock) {
+                        const uint32_t b = a ^ j;
+                        if (b > a) {
+                            const uint32_t x = s_v[a], y = s_v[b];
+                            if ((x > y) == ((a & k) == 0)) {
+                                s_v[a] = y;
+                                s_v[b] = x;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (uint32_t j = threadIdx.x; j < m; j += kHashBlock) src[lo + j] = s_v[j];
+            __syncthreads();
+        }
+        for (uint32_t w = kCrowdChunk; w < c; w <<= 1) {
+            for (uint32_t j = threadIdx.x; j < c; j += kHashBlock) {
+                const uint32_t run = j / w;
+                const uint32_t lo = (run & ~1u) * w;
+                const uint32_t mid = min(lo + w, c), hi = min(lo + 2 * w, c);
+                const uint32_t x = src[j];
+                const uint32_t at = (run & 1u) == 0u ? (j - lo) + count_below(src + mid, hi - mid, x)
+                                                     : (j - mid) + count_below(src + lo, mid - lo, x);
+                dst[lo + at] = x;
+            }
+            __syncthreads();
+            uint32_t* tmp = src;
+            src = dst;
+            dst = tmp;
+        }
+        const uint32_t k = e.x - d.c_off;
+        for (uint32_t r = threadIdx.x; r < c; r += kHashBlock)
+            move_record(src[r], first + r, k, posrho_in, velp_in, posrho_out, velp_out, sorted, skey, id_in, id_out,
+                        div_flags);
+        __syncthreads();  // the next cell reuses the LDS chunk
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1185,7 +1297,7 @@ hipError_t launch_hash(const DeviceBuffers& b, hipStream_t s) {
     if (reset > blocks) blocks = reset;
     if (blocks == 0) blocks = 1;  // still resets the step clock
     hipLaunchKernelGGL(k_hash, dim3((unsigned)blocks), dim3(kHashBlock), 0, s, S, b.posrho[0], b.key, b.rank,
-                       b.counts, b.scan_status, b.scene_done, b.step_start, b.rho_max);
+                       b.counts, b.scan_status, b.scene_done, b.step_start, b.rho_max, b.crowd_count);
     return hipGetLastError();
 }
 
@@ -1207,7 +1319,12 @@ hipError_t launch_gather(const DeviceBuffers& b, hipStream_t s) {
     if (b.sc.blocks256 == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, dim3(b.sc.blocks256), dim3(kHashBlock), 0, s, b.sc, b.slot_tmp, b.key, b.range,
                        b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, b.ids[0], b.ids[1],
-                       b.rho_max + 1);
+                       b.rho_max + 1, b.crowd, b.crowd_count, b.crowd_cap, b.error_flag);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gather_crowded, dim3(kCrowdBlocks), dim3(kHashBlock), 0, s, b.sc, b.crowd, b.crowd_count,
+                       b.crowd_cap, b.slot_tmp, b.range, b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted,
+                       b.skey, b.ids[0], b.ids[1], b.rho_max + 1);
     return hipGetLastError();
 }
 
