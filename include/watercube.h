@@ -188,6 +188,27 @@ WC_API int wc_advect(wc_ctx* ctx, float dt);
  * captured HIP graph per step.  Asynchronous: returns once the work is queued. */
 WC_API int wc_step(wc_ctx* ctx, float dt, uint32_t nsteps);
 
+/* One step without allocating: the step of wc_step (sort, density, force + integrate, in the
+ * reference's pass order) for one scene whose buffer 1 lives in caller-owned device memory, for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops).  No
+ * reference counterpart: the reference allocates its buffers once at setup (Fluid.cpp:139-166).
+ *   p        the scene's parameters (p->num_particles particles, p->grid_res^3 cells)
+ *   posrho   float4 per particle, (x, y, z, density) -- buffer 1's first half, read and rewritten
+ *   velp     float4 per particle, (vx, vy, vz, pressure) -- its second half (both 16-byte aligned)
+ *   status   device word (may be NULL): set to 0 at the start of the step, nonzero if a kernel's
+ *            bounded wait failed (WC_ERR_DEVICE_FAULT's condition); read it after the stream syncs
+ *   stream   hipStream_t to queue on (NULL: the default stream); the call returns once queued
+ * Every intermediate (the sorted buffer 2, cell keys, ranges, scan state) lives in one fixed
+ * device array of the library, WC_NOALLOC_MAX_PARTICLES particles and WC_NOALLOC_MAX_CELLS cells
+ * in size (about 17 MB, held on every GPU the library runs on until the process ends); larger
+ * scenes are refused with WC_ERR_CAPACITY.  The call makes no allocation of any kind.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller.  The neighbour passes search (no neighbour lists);
+ * results are the same bits as wc_step's. */
+#define WC_NOALLOC_MAX_PARTICLES 262144u
+#define WC_NOALLOC_MAX_CELLS 262144u
+WC_API int wc_step_noalloc(const wc_params* p, float dt, void* posrho, void* velp, uint32_t* status, void* stream);
+
 /* ------------------------------------------------------------------------------------------
  * Batched scenes.  A context may hold many independent scenes that advance together: every pass
  * is one launch over all scenes (particles and cell grids concatenated in HBM behind a

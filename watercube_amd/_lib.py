@@ -30,6 +30,8 @@ WC_PRESET_BLOCK_1M = 2
 WC_PRESET_BLOCK_SPHERE_17M = 3
 
 WC_NUM_PASSES = 6
+WC_NOALLOC_MAX_PARTICLES = 262144
+WC_NOALLOC_MAX_CELLS = 262144
 
 # struct Particle (src/core/util.h:29-35) as a numpy record: 32 bytes, offsets 0/12/16/28.
 PARTICLE_DTYPE = np.dtype(
@@ -130,6 +132,7 @@ SIGNATURES = {
     "wc_advect": (C.c_int, [_vp, C.c_float]),
     "wc_gravity_direction": (C.c_int, [_P(C.c_float), _P(C.c_float)]),
     "wc_step": (C.c_int, [_vp, C.c_float, C.c_uint32]),
+    "wc_step_noalloc": (C.c_int, [_P(wc_params), C.c_float, _vp, _vp, _u32p, _vp]),
     "wc_synchronize": (C.c_int, [_vp]),
     "wc_create_batch": (C.c_int, [_P(wc_params), C.c_uint32, _P(_vp)]),
     "wc_num_scenes": (C.c_int, [_vp]),

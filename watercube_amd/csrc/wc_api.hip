@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -531,6 +532,58 @@ int download_u32(wc_ctx* c, const uint32_t* dev, uint32_t* host, uint32_t n) {
 
 const char* const kPassNames[WC_NUM_PASSES] = {"hash", "scan", "place", "gather", "density", "update"};
 
+// ------------------------------------------------------------------------------------------
+// Scratch of the allocation-free step (wc_step_noalloc): one fixed device array per GPU, laid
+// out as the buffers a one-scene context allocates, for at most kNoAllocParticles particles and
+// kNoAllocCells cells.  Host code never uses the array's name as an address: its per-device
+// address comes from hipGetSymbolAddress on the first call on each device.
+// ------------------------------------------------------------------------------------------
+constexpr size_t kNoAllocParticles = WC_NOALLOC_MAX_PARTICLES;
+constexpr size_t kNoAllocCells = WC_NOALLOC_MAX_CELLS;
+static_assert(kNoAllocCells % wc::kScanTile == 0, "whole scan tiles");
+constexpr size_t kNoAllocTiles = kNoAllocCells / wc::kScanTile;
+constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+// byte offsets in the array
+constexpr size_t kNaHeader = 0;  // error word, rho_max[2], scene_done, step_start, crowded-cell count
+constexpr size_t kNaDesc = kNaHeader + 256;
+constexpr size_t kNaTables = kNaDesc + align256(sizeof(SceneDesc));
+constexpr size_t kNaPosrho = kNaTables + 256;
+constexpr size_t kNaVelp = kNaPosrho + align256((kNoAllocParticles + wc::kRecordPad) * sizeof(float4));
+constexpr size_t kNaKey = kNaVelp + align256((kNoAllocParticles + wc::kRecordPad) * sizeof(float4));
+constexpr size_t kNaRank = kNaKey + align256(kNoAllocParticles * 4);
+constexpr size_t kNaSlot = kNaRank + align256(kNoAllocParticles * 4);
+constexpr size_t kNaSorted = kNaSlot + align256(kNoAllocParticles * 4);
+constexpr size_t kNaSkey = kNaSorted + align256(kNoAllocParticles * 4);
+constexpr size_t kNaCounts = kNaSkey + align256(kNoAllocParticles * 4);
+constexpr size_t kNaRange = kNaCounts + align256(kNoAllocCells * 4);
+constexpr size_t kNaStatus = kNaRange + align256(kNoAllocCells * 8);
+constexpr size_t kNaCrowd = kNaStatus + align256(kNoAllocTiles * 8);
+constexpr size_t kNoAllocBytes = kNaCrowd + align256((kNoAllocParticles / (wc::kGatherDirectMax + 1) + 1) * 8);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// neighbour-pass variant of the allocation-free step: 64 threads, 32 list entries, 4 slots in
+// flight, a drain per dx plane (a context's default before tuning)
+constexpr int kNoAllocVariant = 2;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// The scratch array's address on the current device `dev`, looked up once per device.
+int scratch_on_device(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(WC_ERR_NO_DEVICE, "device %d outside [0, %d)", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(WC_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[dev];
+    return WC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -739,6 +792,76 @@ WC_API int wc_step(wc_ctx* c, float dt, uint32_t nsteps) {
     int rc = set_all_dt(c, &dt, true);
     if (rc) return rc;
     return run_steps(c, nsteps);
+}
+
+WC_API int wc_step_noalloc(const wc_params* p, float dt, void* posrho, void* velp, uint32_t* status, void* stream) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    const uint64_t R = (uint64_t)p->grid_res;
+    const uint64_t ncells = R * R * R;
+    const uint32_t n = p->num_particles;
+    if (n > kNoAllocParticles || ncells > kNoAllocCells)
+        return fail(WC_ERR_CAPACITY, "wc_step_noalloc holds at most %zu particles and %zu cells; got %u and %llu",
+                    kNoAllocParticles, kNoAllocCells, n, (unsigned long long)ncells);
+    if (n == 0) return WC_OK;
+    if (!posrho || !velp) return fail(WC_ERR_INVALID_ARG, "NULL particle buffer");
+    if ((reinterpret_cast<uintptr_t>(posrho) | reinterpret_cast<uintptr_t>(velp)) & 15u)
+        return fail(WC_ERR_INVALID_ARG, "particle buffers must be 16-byte aligned");
+    int dev = 0;
+    WC_HIP(hipGetDevice(&dev));
+    unsigned char* base = nullptr;
+    rc = scratch_on_device(dev, &base);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    SceneDesc d;
+    memset(&d, 0, sizeof(d));
+    d.c = wc::make_consts(*p);
+    d.dt = dt;
+    const uint32_t tiles = ceil_div(ncells, wc::kScanTile);
+    const uint32_t tables[8] = {0u, ceil_div(n, 64), 0u, ceil_div(n, 128), 0u, ceil_div(n, 256), 0u, tiles};
+    DeviceBuffers b{};
+    b.posrho[0] = static_cast<float4*>(posrho);
+    b.velp[0] = static_cast<float4*>(velp);
+    b.posrho[1] = reinterpret_cast<float4*>(base + kNaPosrho);
+    b.velp[1] = reinterpret_cast<float4*>(base + kNaVelp);
+    b.key = reinterpret_cast<uint32_t*>(base + kNaKey);
+    b.rank = reinterpret_cast<uint32_t*>(base + kNaRank);
+    b.slot_tmp = reinterpret_cast<uint32_t*>(base + kNaSlot);
+    b.sorted = reinterpret_cast<uint32_t*>(base + kNaSorted);
+    b.skey = reinterpret_cast<uint32_t*>(base + kNaSkey);
+    b.counts = reinterpret_cast<uint32_t*>(base + kNaCounts);
+    b.range = reinterpret_cast<uint2*>(base + kNaRange);
+    b.scan_status = reinterpret_cast<unsigned long long*>(base + kNaStatus);
+    b.error_flag = status ? status : reinterpret_cast<uint32_t*>(base + kNaHeader);
+    b.rho_max = reinterpret_cast<uint32_t*>(base + kNaHeader + 8);
+    b.scene_done = reinterpret_cast<unsigned long long*>(base + kNaHeader + 16);
+    b.step_start = reinterpret_cast<unsigned long long*>(base + kNaHeader + 24);
+    b.crowd_count = reinterpret_cast<uint32_t*>(base + kNaHeader + 32);
+    b.crowd = reinterpret_cast<uint2*>(base + kNaCrowd);
+    b.crowd_cap = wc::crowd_capacity(kNoAllocParticles);
+    SceneDesc* d_desc = reinterpret_cast<SceneDesc*>(base + kNaDesc);
+    uint32_t* d_tables = reinterpret_cast<uint32_t*>(base + kNaTables);
+    wc::Scenes& v = b.sc;
+    v.desc = d_desc;
+    v.blk64 = d_tables;
+    v.blk128 = d_tables + 2;
+    v.blk256 = d_tables + 4;
+    v.tiles = d_tables + 6;
+    v.num_scenes = 1;
+    v.total_particles = n;
+    v.total_tiles = tiles;
+    v.blocks64 = tables[1];
+    v.blocks128 = tables[3];
+    v.blocks256 = tables[5];
+    WC_HIP(wc::launch_noalloc_setup(d, tables, d_desc, d_tables, b.counts, (uint32_t)ncells, b.error_flag, s));
+    WC_HIP(wc::launch_hash(b, s));
+    WC_HIP(wc::launch_scan(b, s));
+    WC_HIP(wc::launch_place(b, s));
+    WC_HIP(wc::launch_gather(b, s));
+    WC_HIP(wc::launch_density(b, kNoAllocVariant, false, s));
+    WC_HIP(wc::launch_update(b, kNoAllocVariant, false, s));
+    return WC_OK;
 }
 
 WC_API int wc_set_scene_dt(wc_ctx* c, const float* dt, uint32_t num_scenes) {

@@ -166,6 +166,10 @@ hipError_t launch_advect(const DeviceBuffers& b, float dt, hipStream_t stream);
 // After the host changed particle buffer 2 or the parameters: recompute buffer 2's cell keys,
 // mark the neighbour lists overflowed and the force pass's division operands unchecked.
 hipError_t launch_invalidate(const DeviceBuffers& b, hipStream_t stream);
+// Allocation-free step: writes one scene's table (d, and the 8 words blk64[0..1], blk128[0..1],
+// blk256[0..1], tiles[0..1]) into scratch memory and zeroes counts[0, ncells) and *error_flag.
+hipError_t launch_noalloc_setup(const SceneDesc& d, const uint32_t tables_host[8], SceneDesc* desc, uint32_t* tables,
+                                uint32_t* counts, uint32_t ncells, uint32_t* error_flag, hipStream_t stream);
 hipError_t launch_split_records(const wc_particle* rec, float4* posrho, float4* velp, uint32_t n,
                                 hipStream_t stream);
 hipError_t launch_join_records(const float4* posrho, const float4* velp, wc_particle* rec, uint32_t n,

@@ -1267,6 +1267,27 @@ __global__ __launch_bounds__(kHashBlock) void k_invalidate(Scenes S, const float
 }
 
 // ------------------------------------------------------------------------------------------
+// Allocation-free step: the scene's constants and block tables go into the library's fixed
+// scratch array (from kernel arguments, so nothing is copied from host memory), and what a step
+// expects to start at zero is cleared: the cell counts (a step whose scan failed leaves some
+// behind) and the failure word.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kHashBlock) void k_noalloc_setup(SceneDesc d, uint4 t0, uint4 t1,
+                                                              SceneDesc* __restrict__ desc,
+                                                              uint32_t* __restrict__ tables,
+                                                              uint32_t* __restrict__ counts, uint32_t ncells,
+                                                              uint32_t* __restrict__ error_flag) {
+    const uint32_t g = blockIdx.x * kHashBlock + threadIdx.x;
+    if (g == 0) {
+        *desc = d;
+        reinterpret_cast<uint4*>(tables)[0] = t0;
+        reinterpret_cast<uint4*>(tables)[1] = t1;
+        *error_flag = 0u;
+    }
+    for (uint32_t i = g; i < ncells; i += gridDim.x * kHashBlock) counts[i] = 0u;
+}
+
+// ------------------------------------------------------------------------------------------
 // Record <-> split-half conversion.
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kHashBlock) void k_split(const float4* __restrict__ rec, float4* __restrict__ posrho,
@@ -1452,6 +1473,18 @@ hipError_t launch_closure_update(const DeviceBuffers& b, float dt, const float* 
                            b.posrho[1], b.velp[1], b.skey, b.range, b.posrho[0], b.velp[0], b.scene_done, b.ids[1],
                            b.ids[0], dt, reinterpret_cast<const float4*>(corr), corr_scale,
                            reinterpret_cast<float4*>(acc_out), nullptr, nullptr, substeps, b.rho_max + 1, backup);
+    return hipGetLastError();
+}
+
+hipError_t launch_noalloc_setup(const SceneDesc& d, const uint32_t tables_host[8], SceneDesc* desc,
+                                uint32_t* tables, uint32_t* counts, uint32_t ncells, uint32_t* error_flag,
+                                hipStream_t s) {
+    const uint4 t0 = make_uint4(tables_host[0], tables_host[1], tables_host[2], tables_host[3]);
+    const uint4 t1 = make_uint4(tables_host[4], tables_host[5], tables_host[6], tables_host[7]);
+    unsigned blocks = blocks_for(ncells, kHashBlock);
+    blocks = blocks < 1 ? 1 : (blocks > 256 ? 256 : blocks);
+    hipLaunchKernelGGL(k_noalloc_setup, dim3(blocks), dim3(kHashBlock), 0, s, d, t0, t1, desc, tables, counts, ncells,
+                       error_flag);
     return hipGetLastError();
 }
 
