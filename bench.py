@@ -113,10 +113,9 @@ def cpu_baseline(p, parts, dt, budget_s: float = 12.0, max_steps: int = 60) -> d
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{done} steps of {WORKLOAD} ({n} particles, dt={dt}) from the same initial state, "
-                  f"C restatement -O3 with OpenMP over particles for density/update, {elapsed:.1f} s",
-        "single_thread": {"value": n * one_done / one_elapsed, "unit": "particle-steps/s", "cores": 1,
-                          "sample": f"the next {one_done} steps on one core, {one_elapsed:.1f} s"},
+        "sample": f"first {done} steps of {WORKLOAD}, C restatement -O3, OpenMP over particles, {elapsed:.1f} s",
+        "single_thread": {"value": n * one_done / one_elapsed, "cores": 1,
+                          "sample": f"next {one_done} steps on one core, {one_elapsed:.1f} s"},
     }
 
 
@@ -356,6 +355,12 @@ def main() -> int:
 
     pass_kernels = {"hash": ["hash"], "scan": ["scan"], "scatter": ["place", "gather"], "density": ["density"],
                     "update": ["update"]}
+    # per pass: algorithmic bytes and rocprofv3-measured HBM bytes (committed summary of the same
+    # steps) over the live per-launch duration, in GB/s
+    pass_gbs = {k: [round(algo[k] / (v * 1e-3) / 1e9, 1),
+                    None if measured(pass_kernels[k]) is None else round(measured(pass_kernels[k]) / (v * 1e-3) / 1e9, 1)]
+                for k, v in pass_ms.items() if v > 0}
+    print(f"# per-pass GB/s [algorithmic, measured]: {json.dumps(pass_gbs)}", file=sys.stderr, flush=True)
     traffic = measured([dom])
     valu = read_valu(dom)
     valu_frac = None if valu is None else valu / dom_s / VALU_PEAK_WAVE_INSTR_PER_S
@@ -380,12 +385,6 @@ def main() -> int:
         "simulated_s_per_s": dt * args.steps / elapsed,
         "pass_ms": per_launch_ms,
         "pass_ms_steps": prof_steps,
-        # per pass: algorithmic bytes and rocprofv3-measured HBM bytes (committed summary of the
-        # same steps) over the live per-launch duration, in GB/s
-        "pass_gbs": {k: {"algorithmic": algo[k] / (v * 1e-3) / 1e9,
-                         "measured": None if measured(pass_kernels[k]) is None
-                         else measured(pass_kernels[k]) / (v * 1e-3) / 1e9}
-                     for k, v in pass_ms.items() if v > 0},
         "roofline": {
             "kernel": dom,
             # the density pass is bound by its instruction issue, not by HBM: `frac` is its
@@ -399,23 +398,24 @@ def main() -> int:
             "hbm_frac": achieved / HBM_PEAK_GBS,
             "valu_frac": valu_frac,
             "traffic": traffic,
-            "traffic_source": f"profiles/traffic_{WORKLOAD}.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
-                              f"over steps 521-570, calibrated on a 32 MB copy",
+            "traffic_source": f"profiles/traffic_{WORKLOAD}.json (steps 521-570)",
             "duration_us": per_launch_ms[dom] * 1e3,
             "duration_steps": prof_steps,
             "algorithmic_bytes": algo[dom],
             "overhead_bytes": extra.get(dom),
-            "neighbour_list_entries_per_particle": list_entries,
-            "valu_wave_instr_per_launch": valu,
-            "valu_peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
+            "list_entries": list_entries,
+            "valu_wave_instr": valu,
         },
     }
     if args.batch_steps > 0:
-        out["batched"] = batched_section(args.batch_steps)
-        print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
+        bt = batched_section(args.batch_steps)
+        print(f"# batched: {bt}", file=sys.stderr, flush=True)
+        out["batched"] = {k: bt[k] for k in ("workload", "particles", "value", "ms_per_step", "scene_latency_ms_p50",
+                                             "scene_latency_ms_p99")}
     if args.scale_steps > 0:
-        out["scale"] = scale_section(args.scale_steps)
-        print(f"# scale: {out['scale']}", file=sys.stderr, flush=True)
+        sc = scale_section(args.scale_steps)
+        print(f"# scale: {sc}", file=sys.stderr, flush=True)
+        out["scale"] = {k: sc[k] for k in ("workload", "particles", "value", "ms_per_step")}
     weights = Path(args.weights)
     if weights.exists():
         learned = learned_section(p, parts, dt, weights, horizon=args.learned_horizon)
@@ -444,27 +444,27 @@ def _rounded(x, digits: int = 4):
 
 
 def learned_summary(res: dict) -> dict:
-    """The learned comparison's headline figures: for each K and method the density errors,
-    fallbacks and speed-up over the plain dt run; the one-step error ratios; the network's
-    matrix-core use; the fastest method within 1 % density error."""
-    keep = ("mean_density_err", "p99_density_err", "max_density_err", "speedup_vs_plain", "fallbacks")
-    out = {"horizon_steps": res.get("horizon_steps"), "weights": res.get("weights"),
-           "best_within_1pct": res.get("best_within_1pct")}
+    """The learned comparison's headline figures: per K and method [mean, max density error,
+    speed-up over the plain dt run, fallbacks]; the one-step velocity-error ratio of the learned
+    step to the plain large step; the fastest method within 1 % density error; the network's
+    matrix-core use (the whole comparison goes to stderr)."""
+    out = {"horizon_steps": res.get("horizon_steps")}
     for key, entry in res.items():
         if not (key.startswith("K") and isinstance(entry, dict)):
             continue
         k = {}
         for m, e in entry.items():
             if m == "one_step":
-                k["one_step_ratio"] = e["ratio_to_plain_large"]
-            elif isinstance(e, dict):
-                k[m] = {f: e[f] for f in keep if f in e}
-                if "one_step_ratio" in e:
-                    k[m]["one_step_ratio"] = e["one_step_ratio"]
+                k["one_step_vel_ratio"] = e["ratio_to_plain_large"]["learned"]["vel"]
+            elif isinstance(e, dict) and "mean_density_err" in e:
+                k[m] = [e["mean_density_err"], e["max_density_err"], e["speedup_vs_plain"], e.get("fallbacks", 0)]
         out[key] = k
+    best = res.get("best_within_1pct") or {}
+    out["best_within_1pct"] = {k: (None if v is None else [v["method"], v["K"], v["speedup_vs_plain"]])
+                               for k, v in best.items()}
     for key in ("inference", "inference_from_lists"):
         if key in res:
-            out[key] = {f: res[key][f] for f in ("ms", "mfma_utilisation") if f in res[key]}
+            out[f"{key}_mfma_utilisation"] = res[key]["mfma_utilisation"]
     return out
 
 
