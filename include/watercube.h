@@ -389,6 +389,9 @@ typedef struct wc_closure_stats {
     uint32_t fallbacks;      /* of them replaced by K plain steps */
     double wall_ms;          /* device time of the whole run */
     double max_density_jump; /* largest rise of the density maximum over one learned step, / rho0 */
+    double max_accepted_jump; /* largest rise of the density maximum over one accepted learned step,
+                               * / rho0: the judged quantity, at most judge_tol by construction */
+    double peak_density;      /* largest density maximum of the states the run kept, / rho0 */
 } wc_closure_stats;
 
 /* nmacro learned steps of K * dt.  Each: sort, density, closure inputs, network, force pass with

@@ -101,7 +101,7 @@ class wc_device_view(C.Structure):
 
 class wc_closure_stats(C.Structure):
     _fields_ = [("macro_steps", C.c_uint32), ("fallbacks", C.c_uint32), ("wall_ms", C.c_double),
-                ("max_density_jump", C.c_double)]
+                ("max_density_jump", C.c_double), ("max_accepted_jump", C.c_double), ("peak_density", C.c_double)]
 
 
 class wc_tuning(C.Structure):

@@ -1548,6 +1548,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
     float prev_max = 0.0f;
     rc = sort_density();
     if (!rc) rc = read_rho_max(c, &prev_max);
+    stats->peak_density = prev_max / rho0;
     for (int slot = 0; slot < 2 && !rc && nmacro > 0; ++slot) {
         WC_HIP(hipEventCreateWithFlags(&mg.done[slot], hipEventDisableTiming));
         WC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1611,6 +1612,8 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
             if (!rc) rc = read_rho_max(c, &now);
             if (!rc && ahead) rc = launch(m + 1);
         }
+        if (jump <= judge_tol && (double)jump > stats->max_accepted_jump) stats->max_accepted_jump = jump;
+        if ((double)now / rho0 > stats->peak_density) stats->peak_density = (double)now / rho0;
         prev_max = now;
     }
     if (!rc) {
