@@ -543,6 +543,7 @@ def main(argv=None) -> int:
     e.add_argument("--ks", default="2,4,8", help="step multipliers K to compare, comma-separated")
     e.add_argument("--exact", action="store_true", help="also the exact-correction upper bound")
     e.add_argument("--raw", action="store_true", help="each particle's own correction (no averaging)")
+    e.add_argument("--judge-tol", type=float, default=0.01, help="largest accepted rise of the density maximum / rho0")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -566,7 +567,8 @@ def main(argv=None) -> int:
     else:
         net = ClosureNet(max_batch=1024).load(args.weights)
         ks = tuple(int(k) for k in args.ks.split(","))
-        log(json.dumps(evaluate(net, horizon=args.horizon, ks=ks, smooth=not args.raw, exact=args.exact, log=log)))
+        log(json.dumps(evaluate(net, horizon=args.horizon, ks=ks, judge_tol=args.judge_tol, smooth=not args.raw,
+                                exact=args.exact, log=log)))
     return 0
 
 

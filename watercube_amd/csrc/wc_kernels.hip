@@ -835,7 +835,9 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                                                   float4* __restrict__ acc_out, const uint32_t* __restrict__ nlist,
                                                   const uint32_t* __restrict__ ncount, float substeps,
                                                   const uint32_t* __restrict__ div_flags, RecordBackup backup) {
-    __shared__ uint32_t s_list[CAP * BLOCK];
+    // with neighbour lists the pass keeps no LDS (a particle whose list overflowed searches
+    // without one, below), so that its occupancy is bound by registers alone
+    __shared__ uint32_t s_list[LIST ? 1 : CAP * BLOCK];
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = scene_of(scene_table<BLOCK>(S), S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
@@ -934,9 +936,31 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
                 }
             }
         }
-        if (searched)
-            search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z, posrho_in,
-                                                              range + d.c_off, c, list, drain);
+        if (searched) {
+            if constexpr (LIST) {
+                // a list that overflowed (more than kNeighbourListMax neighbours), or lists older than
+                // the last sort: every candidate is tested and an accepted one added at once, in
+                // enumeration order, one slot at a time (rare, so no LDS list is kept for it)
+                auto visit = [&](uint32_t start, uint32_t end, auto own) {
+                    for (uint32_t u = start; u < end; ++u) {
+                        if (decltype(own)::value && u == t) continue;
+                        const float4 q = posrho_in[u];
+                        const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                        const float d2 = (rx * rx + ry * ry) + rz * rz;
+                        if (d2 >= c.cut2) continue;  // dist >= h; a NaN distance is accepted, as in the search
+                        const float4 qv = velp_in[u];
+                        const float4 cj = load_corr(u);
+                        const float dist = f.template add<FAST>(c, p, pv, q, qv);
+                        add_corr(dist, q, cj);
+                    }
+                };
+                auto nothing = []() {};
+                for_each_neighbour_cell(skey[t], range + d.c_off, c, d.p_off + c.n, visit, nothing);
+            } else {
+                search_neighbours<BLOCK, CAP, CHUNK, PLANE_DRAIN>(skey[t], t, d.p_off + c.n, p.x, p.y, p.z,
+                                                                  posrho_in, range + d.c_off, c, list, drain);
+            }
+        }
     };
     if (div_flags[0] == 0u)
         accumulate(std::true_type{});
