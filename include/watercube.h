@@ -390,15 +390,16 @@ typedef struct wc_closure_stats {
     double wall_ms;          /* device time of the whole run */
     double max_density_jump; /* largest rise of the density maximum over one learned step, / rho0 */
     double max_accepted_jump; /* largest rise of the density maximum over one accepted learned step,
-                               * / rho0: the judged quantity, at most judge_tol by construction */
+                               * / rho0: the judged quantity, at most K * judge_tol by construction */
     double peak_density;      /* largest density maximum of the states the run kept, / rho0 */
 } wc_closure_stats;
 
 /* nmacro learned steps of K * dt.  Each: sort, density, closure inputs, network, force pass with
  * the correction.  Judge: after every learned step the new state's density maximum may exceed
- * the one before the step by at most judge_tol * rho0; otherwise the step is replaced by K plain
- * steps of dt from the saved state.  net = NULL takes the same judged large steps with no
- * correction (no closure inputs, no network). */
+ * the one before the step by at most K * judge_tol * rho0 (judge_tol per small step of dt the
+ * learned step replaces); otherwise the step is replaced by K plain steps of dt from the saved
+ * state.  net = NULL takes the same judged large steps with no correction (no closure inputs,
+ * no network). */
 WC_API int wc_closure_run(wc_ctx* ctx, wc_mlp* net, float dt, uint32_t K, uint32_t nmacro, float judge_tol,
                           wc_closure_stats* stats);
 

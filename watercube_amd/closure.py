@@ -448,7 +448,8 @@ def evaluate(net: ClosureNet | None, workload: str = "block_1m", horizon: int = 
       compresses the fluid further than the small-dt run did shows up here, whatever happens to
       individual particles;
     * max_accepted_jump (judged runs): the largest rise of the density maximum over one accepted
-      learned step, / rho0 -- the judge's own measure, at most judge_tol by construction;
+      learned step, / rho0 -- the judge's own measure, at most K * judge_tol by construction
+      (judge_tol is per small step of dt);
       max_step_jump (plain runs): the same rise over every step, for comparison;
     * mean / p99 / max of |rho - rho_ref| / rho0 per particle at the end of the horizon, particles
       matched by identity with the small-dt run (a trajectory-divergence measure);
@@ -543,7 +544,8 @@ def main(argv=None) -> int:
     e.add_argument("--ks", default="2,4,8", help="step multipliers K to compare, comma-separated")
     e.add_argument("--exact", action="store_true", help="also the exact-correction upper bound")
     e.add_argument("--raw", action="store_true", help="each particle's own correction (no averaging)")
-    e.add_argument("--judge-tol", type=float, default=0.01, help="largest accepted rise of the density maximum / rho0")
+    e.add_argument("--judge-tol", type=float, default=0.01,
+                   help="largest accepted rise of the density maximum / rho0, per small step of dt")
     args = ap.parse_args(argv)
 
     def log(msg):

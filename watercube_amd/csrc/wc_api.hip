@@ -1505,6 +1505,8 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
     rc = set_device(c);
     if (rc) return rc;
     memset(stats, 0, sizeof(*stats));
+    // the judge's allowance for one learned step: judge_tol per small step of dt it replaces
+    const float tol = judge_tol * (float)K;
     const uint32_t n = c->total_particles;
     const float rho0 = c->desc[0].c.rho0;
     const float dt_large = dt * (float)K;
@@ -1599,7 +1601,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
             if (rc) break;
             queued = m + 1;
         }
-        const bool ahead = m + 1 < nmacro && !(last_jump > 0.5f * judge_tol);
+        const bool ahead = m + 1 < nmacro && !(last_jump > 0.5f * tol);
         if (ahead) {
             rc = launch(m + 1);
             if (rc) break;
@@ -1616,7 +1618,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         const float jump = (now - prev_max) / rho0;
         last_jump = jump;
         if ((double)jump > stats->max_density_jump) stats->max_density_jump = jump;
-        if (!(jump <= judge_tol)) {  // NaN fails too
+        if (!(jump <= tol)) {  // NaN fails too
             stats->fallbacks += 1;
             WC_HIP(hipMemcpyAsync(c->b.posrho[0], c->cl_backup[slot][0], (size_t)n * sizeof(float4),
                                   hipMemcpyDeviceToDevice, c->stream));
@@ -1630,7 +1632,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
             if (!rc) rc = read_rho_max(c, &now);
             if (!rc && ahead) rc = launch(m + 1);
         }
-        if (jump <= judge_tol && (double)jump > stats->max_accepted_jump) stats->max_accepted_jump = jump;
+        if (jump <= tol && (double)jump > stats->max_accepted_jump) stats->max_accepted_jump = jump;
         if ((double)now / rho0 > stats->peak_density) stats->peak_density = (double)now / rho0;
         prev_max = now;
     }
