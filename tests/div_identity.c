@@ -8,6 +8,7 @@
  *   divisors:    a density rho in [2^-40, 2^40], 2 rho, and d = dist + 1e-16 in [1e-16, 2^42].
  *
  *   div_identity exhaustive B0 B1    every significand of a against divisor significands [B0, B1)
+ *   div_identity sampled N SEED      every significand of a against N random divisor significands
  *   div_identity random N SEED       N random operand pairs drawn from the ranges above
  *
  * Prints the number of pairs and of mismatches (bitwise, except that a = -0 may give +0, which the
@@ -55,21 +56,37 @@ static float rnd(int lo, int hi) {
     return from_bits(((r >> 63) ? 0x80000000u : 0u) | ((uint32_t)(e + 127) << 23) | m);
 }
 
+/* mismatches of a / b over every significand of a in [1, 2) */
+static long long all_numerators(uint32_t mb) {
+    const float b = from_bits(0x3f800000u | mb);
+    const float y = 1.0f / b;
+    long long bad = 0;
+    for (uint32_t ma = 0; ma < (1u << 23); ++ma) {
+        const float a = from_bits(0x3f800000u | ma);
+        bad += !same(quot(a, b, y), a / b, a);
+    }
+    return bad;
+}
+
 int main(int argc, char** argv) {
     if (argc == 4 && !strcmp(argv[1], "exhaustive")) {
         const uint32_t lo = (uint32_t)strtoul(argv[2], 0, 0), hi = (uint32_t)strtoul(argv[3], 0, 0);
-        long long bad = 0, total = 0;
-#pragma omp parallel for reduction(+ : bad, total) schedule(dynamic, 4)
-        for (uint32_t mb = lo; mb < hi; ++mb) {
-            const float b = from_bits(0x3f800000u | mb);
-            const float y = 1.0f / b;
-            for (uint32_t ma = 0; ma < (1u << 23); ++ma) {
-                const float a = from_bits(0x3f800000u | ma);
-                bad += !same(quot(a, b, y), a / b, a);
-                ++total;
-            }
-        }
-        printf("%lld %lld\n", total, bad);
+        long long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(dynamic, 4)
+        for (uint32_t mb = lo; mb < hi; ++mb) bad += all_numerators(mb);
+        printf("%lld %lld\n", (long long)(hi - lo) << 23, bad);
+        return bad != 0;
+    }
+    if (argc == 4 && !strcmp(argv[1], "sampled")) {
+        const int n = atoi(argv[2]);
+        state = strtoull(argv[3], 0, 0);
+        uint32_t* mbs = malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+        for (int i = 0; i < n; ++i) mbs[i] = (uint32_t)(next() >> 41); /* 23 random bits */
+        long long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(dynamic, 4)
+        for (int i = 0; i < n; ++i) bad += all_numerators(mbs[i]);
+        free(mbs);
+        printf("%lld %lld\n", (long long)n << 23, bad);
         return bad != 0;
     }
     if (argc == 4 && !strcmp(argv[1], "random")) {
@@ -95,6 +112,6 @@ int main(int argc, char** argv) {
         printf("%lld %lld\n", 4 * n, bad);
         return bad != 0;
     }
-    fprintf(stderr, "usage: div_identity exhaustive B0 B1 | random N SEED\n");
+    fprintf(stderr, "usage: div_identity exhaustive B0 B1 | sampled N SEED | random N SEED\n");
     return 2;
 }

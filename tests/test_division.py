@@ -9,6 +9,8 @@ with the CPU's IEEE division and fma:
 
 * exhaustively, every significand of a against divisor significands near 1, near sqrt(2) and near 2,
   where the reciprocal's rounding error is largest relative to the quotient's;
+* every significand of a against 4096 random divisor significands (2^35 pairs; the quotient's
+  rounding depends only on the two significands, so this samples the whole divisor range);
 * on random operands drawn the way the kernel forms them (sums and differences of in-range values;
   r / d with d = |r| + 1e-16).
 
@@ -47,6 +49,12 @@ def _run(exe, *args):
 def test_exhaustive_significands(div_identity, lo, hi):
     rc, total, bad = _run(div_identity, "exhaustive", hex(lo), hex(hi))
     assert total == (hi - lo) << 23
+    assert bad == 0 and rc == 0
+
+
+def test_sampled_divisor_significands(div_identity):
+    rc, total, bad = _run(div_identity, "sampled", 4096, 20240611)
+    assert total == 4096 << 23
     assert bad == 0 and rc == 0
 
 
