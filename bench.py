@@ -193,8 +193,9 @@ def learned_section(p, parts, dt, weights: Path, horizon: int = 500, reps: int =
     correction every network is bounded by.  For each: simulated time per wall second, the peak
     density's excess over the reference run (the density error the methods are ranked by), the
     judge's measure and per-particle errors (closure.evaluate).  Then the fastest method within 1 %
-    peak-density error, and the inference kernel's matrix-core utilisation (116,224 FLOP per
-    particle over its measured duration, against the dense bf16 peak)."""
+    peak-density error, and the matrix-core utilisation of the inference kernel (116,224 FLOP per
+    particle over its measured duration, against the dense bf16 peak) and of a training step
+    (298,752 FLOP per row: forward, weight gradients, input gradients of layers 2-4)."""
     import ctypes as C
 
     from watercube_amd import Fluid
@@ -241,6 +242,25 @@ def learned_section(p, parts, dt, weights: Path, horizon: int = 500, reps: int =
     fused_tflops = flops / (fused_ms * 1e-3) / 1e12
     res["inference_from_lists"] = {"particles": n, "ms": fused_ms, "clock": "host, per call, selection included",
                                    "tflops": fused_tflops, "mfma_utilisation": fused_tflops / MFMA_BF16_PEAK_TFLOPS}
+    # training: forward, backward and Adam through the layer-by-layer GEMM kernels, on random rows
+    rng = np.random.default_rng(0)
+    res["training"] = {}
+    for batch in (8192, 65536):
+        tnet = closure.ClosureNet(max_batch=batch)
+        xt = closure.DeviceArray((batch, closure.NFEAT), np.uint16)
+        tt = closure.DeviceArray((batch, 4), np.float32)
+        xf = rng.standard_normal((batch, closure.NFEAT)).astype(np.float32)
+        xf[:, 195:] = 0.0
+        xt.upload((xf.view(np.uint32) >> 16).astype(np.uint16))
+        tt.upload(rng.standard_normal((batch, 4)).astype(np.float32) * 0.1)
+        t_ms = tnet.profile_train(xt, tt, batch, lr=1e-4, reps=reps)
+        tnet.close()
+        xt.free()
+        tt.free()
+        t_tflops = closure.mlp_train_flops_per_particle() * batch / (t_ms * 1e-3) / 1e12
+        res["training"][str(batch)] = {"ms_per_step": t_ms, "tflops": t_tflops,
+                                       "mfma_utilisation": t_tflops / MFMA_BF16_PEAK_TFLOPS}
+    res["training"]["flop_per_row"] = closure.mlp_train_flops_per_particle()
     res["weights"] = str(weights.relative_to(ROOT)) if weights.is_relative_to(ROOT) else str(weights)
 
     # candidates: (method, K, entry); the plain dt run is the reference (no error by definition)
@@ -435,7 +455,8 @@ def learned_summary(res: dict) -> dict:
     """The learned comparison's headline figures: per K and method [peak-density excess over the
     plain dt run, speed-up over it, fallbacks]; the exact-correction bound's excess; the one-step
     velocity-error ratio of the learned step to the plain large step; the fastest methods within
-    1 % peak-density error; the network's matrix-core use (the whole comparison goes to stderr)."""
+    1 % peak-density error; the network's matrix-core use in inference and in training (the whole
+    comparison goes to stderr)."""
     out = {"horizon_steps": res.get("horizon_steps")}
     for key, entry in res.items():
         if not (key.startswith("K") and isinstance(entry, dict)):
@@ -455,6 +476,9 @@ def learned_summary(res: dict) -> dict:
     for key in ("inference", "inference_from_lists"):
         if key in res:
             out[f"{key}_mfma_utilisation"] = res[key]["mfma_utilisation"]
+    if "training" in res:
+        out["training_mfma_utilisation"] = {b: e["mfma_utilisation"] for b, e in res["training"].items()
+                                            if isinstance(e, dict)}
     return out
 
 

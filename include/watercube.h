@@ -342,6 +342,11 @@ WC_API int wc_mlp_gradient(wc_mlp* net, const void* x_dev, const void* t_dev, ui
 WC_API int wc_mlp_train_step(wc_mlp* net, const void* x_dev, const void* t_dev, uint32_t n, float lr, double* loss);
 /* Average duration of the fused inference kernel over reps launches (HIP events). */
 WC_API int wc_mlp_profile_predict(wc_mlp* net, const void* x_dev, void* y_dev, uint32_t n, uint32_t reps, double* ms);
+/* Average duration of one training step (forward, backward, Adam; the same kernels as
+ * wc_mlp_train_step, without the loss read-back) over reps steps (HIP events).  The steps update
+ * the parameters. */
+WC_API int wc_mlp_profile_train(wc_mlp* net, const void* x_dev, const void* t_dev, uint32_t n, float lr, uint32_t reps,
+                                double* ms);
 
 /* Device memory for the closure's inputs, targets and outputs. */
 WC_API int wc_device_alloc(size_t bytes, void** out);

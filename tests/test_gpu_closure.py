@@ -81,6 +81,27 @@ def test_training_fits_a_teacher_network():
     assert last < 0.1 * first
 
 
+def test_timed_training_steps_are_training_steps():
+    """wc_mlp_profile_train runs one warm-up step and `reps` timed steps of the same kernels as
+    wc_mlp_train_step: its parameters afterwards equal those of reps + 1 plain training steps.  A
+    batch under 4,096 rows keeps every GEMM unsplit, so each gradient is summed in one order."""
+    rng = np.random.default_rng(2)
+    m = 2048
+    a = ClosureNet(max_batch=m, seed=5)
+    b = ClosureNet(max_batch=m, seed=5)
+    xb, t = _random_batch(rng, m)
+    xd = DeviceArray((m, 208), np.uint16).upload(xb)
+    td = DeviceArray((m, 4), np.float32).upload(t)
+    ms = a.profile_train(xd, td, m, lr=1e-3, reps=3)
+    for _ in range(4):
+        b.train_step(xd.ptr, td.ptr, m, 1e-3)
+    pa, pb = a.params(), b.params()
+    print(f"timed training step: {ms:.4f} ms at batch {m}", flush=True)
+    assert ms > 0
+    np.testing.assert_array_equal(pa, pb)
+    assert np.abs(pa - ClosureNet(max_batch=m, seed=5).params()).max() > 1e-4
+
+
 def _enumeration(oracle, p, pos, counts, offsets, t):
     R = p.grid_res
     cx, cy, cz = oracle.cell(p, pos[t])

@@ -166,6 +166,7 @@ SIGNATURES = {
     "wc_mlp_gradient": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _f32p, _P(C.c_double)]),
     "wc_mlp_train_step": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_float, _P(C.c_double)]),
     "wc_mlp_profile_predict": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _P(C.c_double)]),
+    "wc_mlp_profile_train": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_float, C.c_uint32, _P(C.c_double)]),
     "wc_device_alloc": (C.c_int, [C.c_size_t, _P(_vp)]),
     "wc_device_free": (C.c_int, [_vp]),
     "wc_device_upload": (C.c_int, [_vp, _vp, C.c_size_t]),

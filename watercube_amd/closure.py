@@ -133,6 +133,13 @@ class ClosureNet:
         L.check(L.lib().wc_mlp_profile_predict(self.h, x.ptr, y.ptr, n, reps, C.byref(ms)), "wc_mlp_profile_predict")
         return ms.value
 
+    def profile_train(self, x: DeviceArray, t: DeviceArray, n: int, lr: float = 1e-4, reps: int = 10) -> float:
+        """Average ms of one training step over `reps` steps (they update the parameters)."""
+        ms = C.c_double()
+        L.check(L.lib().wc_mlp_profile_train(self.h, x.ptr, t.ptr, n, C.c_float(lr), reps, C.byref(ms)),
+                "wc_mlp_profile_train")
+        return ms.value
+
     def close(self):
         if self.h:
             L.lib().wc_mlp_destroy(self.h)
@@ -148,6 +155,13 @@ class ClosureNet:
 def mlp_flops_per_particle() -> int:
     """Multiply-adds x 2 of one forward pass, unpadded (195 -> 128 -> 128 -> 128 -> 3)."""
     return 2 * (195 * 128 + 128 * 128 + 128 * 128 + 128 * 3)
+
+
+def mlp_train_flops_per_particle() -> int:
+    """Multiply-adds x 2 of one training step per row, unpadded: the forward pass, every layer's
+    weight gradient, and the input gradients of layers 2-4 (layer 1's is not needed)."""
+    weights = 195 * 128 + 128 * 128 + 128 * 128 + 128 * 3
+    return 2 * (weights + weights + (128 * 128 + 128 * 128 + 128 * 3))
 
 
 # ---------------------------------------------------------------------------------------------

@@ -1367,6 +1367,29 @@ WC_API int wc_mlp_train_step(wc_mlp* h, const void* x_dev, const void* t_dev, ui
     return WC_OK;
 }
 
+WC_API int wc_mlp_profile_train(wc_mlp* h, const void* x_dev, const void* t_dev, uint32_t n, float lr, uint32_t reps,
+                                double* ms) {
+    if (!ms) return fail(WC_ERR_INVALID_ARG, "NULL argument");
+    if (reps == 0) reps = 1;
+    int rc = wc_mlp_train_step(h, x_dev, t_dev, n, lr, nullptr);  // warm-up; checks the arguments
+    if (rc) return rc;
+    EventPair ev;
+    hipError_t err = ev.create();
+    if (err == hipSuccess) err = hipEventRecord(ev.start, h->m.stream);
+    for (uint32_t r = 0; r < reps && err == hipSuccess; ++r) {
+        err = wc::mlp_forward_train(h->m, (const uint16_t*)x_dev, (const float*)t_dev, nullptr, n, h->m.stream);
+        if (err == hipSuccess) err = wc::mlp_backward(h->m, (const uint16_t*)x_dev, n, h->m.stream);
+        if (err == hipSuccess) err = wc::mlp_adam(h->m, lr, h->m.stream);
+    }
+    if (err == hipSuccess) err = hipEventRecord(ev.stop, h->m.stream);
+    if (err == hipSuccess) err = hipEventSynchronize(ev.stop);
+    float t = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, ev.start, ev.stop);
+    if (err != hipSuccess) return fail(WC_ERR_HIP, "timed training: %s", hipGetErrorString(err));
+    *ms = (double)t / reps;
+    return WC_OK;
+}
+
 WC_API int wc_mlp_profile_predict(wc_mlp* h, const void* x_dev, void* y_dev, uint32_t n, uint32_t reps, double* ms) {
     if (!h || !ms || ((!x_dev || !y_dev) && n)) return fail(WC_ERR_INVALID_ARG, "NULL argument");
     if (reps == 0) reps = 1;
