@@ -189,15 +189,21 @@ __global__ void k_cast(const float* __restrict__ w, uint16_t* __restrict__ rowma
 
 // ------------------------------------------------------------------------------------------
 // Fused inference: Y[n x 4] = MLP(X[n x 208]).  Persistent: one workgroup of eight waves per CU
-// copies the three hidden layers' weights into LDS once (bf16, transposed [out][in], each row
+// copies all four layers' weights and biases into LDS once (bf16, transposed [out][in], each row
 // padded by 8 values so that the 16 rows an MFMA operand read touches fall in different banks)
 // and then walks tiles of 128 rows; each wave takes 16 rows of a tile through the four layers on
-// v_mfma_f32_16x16x32_bf16, its activations in an LDS buffer of its own.  Two waves per SIMD, so
-// one wave's SiLU arithmetic overlaps the other's matrix products.  The output layer's weights
-// (8 KB) are read from global memory.
+// v_mfma_f32_16x16x32_bf16.
 //
-// MFMA 16x16x32 bf16 lane layout (wave64, lane l): A operand = A[l % 16][8 (l / 16) + 0..7],
-// B operand = B[8 (l / 16) + 0..7][l % 16], accumulator element r = C[4 (l / 16) + r][l % 16].
+// MFMA 16x16x32 bf16 lane layout (wave64, lane l, c = l % 16, q = l / 16): A operand =
+// A[c][8 q + 0..7], B operand = B[8 q + 0..7][c], accumulator element r = D[4 q + r][c].
+//
+// Each layer computes the transposed product Z^T = W^T H^T: the weights are the A operand (rows
+// of W^T from LDS) and the activations the B operand, so lane (q, c) works on row c.  Output tile
+// j leaves hidden neuron 16 j + 4 q + r of row c in lane (q, c); the next layer's B operand wants
+// 8 values of row c at input positions 32 s + 8 q + t.  Defining position p = 32 s + 8 q + t to
+// hold neuron 32 s + 16 (t / 4) + 4 q + t % 4 -- the lane's own accumulators acc[2 s + t / 4][t % 4]
+// -- makes every activation stay in the lane that produced it: the input columns of layers 2-4 are
+// stored in that order in LDS, and activations never leave registers.
 // ------------------------------------------------------------------------------------------
 struct InferWeights {
     const uint16_t* w1t;  // [128][208]
@@ -214,25 +220,80 @@ __device__ __forceinline__ f32x4 mfma16(uint4 a, uint4 b, f32x4 c) {
 }
 
 constexpr int kLd1 = kMlpInPad + 8;   // LDS row length of layer 1's weights (208 used, 8 zero)
-constexpr int kLdH = kMlpHidden + 8;  // LDS row length of layers 2-3's weights and of activations
+constexpr int kLdH = kMlpHidden + 8;  // LDS row length of layers 2-4's weights
 constexpr int kInferWaves = 8;
 constexpr int kInferRows = 16;                         // rows per wave and tile
 constexpr int kInferTile = kInferWaves * kInferRows;   // rows per tile
 constexpr int kKSteps1 = (kMlpInPad + 31) / 32;        // layer 1's k steps of 32 (the last half used)
-constexpr int kInferLdsHalves = kMlpHidden * kLd1 + 2 * kMlpHidden * kLdH + kInferWaves * kInferRows * kLdH;
-static_assert(kInferLdsHalves * 2 <= 160 * 1024, "weights and activations must fit one CU's LDS");
+constexpr int kKStepsH = kMlpHidden / 32;              // k steps of layers 2-4
+constexpr int kW4Rows = 16;                            // the output layer's A operand rows (3 used)
+constexpr int kInferLdsHalves = kMlpHidden * kLd1 + 2 * kMlpHidden * kLdH + kW4Rows * kLdH;
+constexpr int kInferBias = 3 * kMlpHidden + 4;         // b1 | b2 | b3 | b4[0..3], floats
+static_assert(kInferLdsHalves % 8 == 0, "the biases follow the weights 16-byte aligned");
+static_assert(kInferLdsHalves * 2 + kInferBias * 4 <= 160 * 1024, "weights must fit one CU's LDS");
 static_assert(32 * kKSteps1 <= kLd1 + 8, "layer 1's last k step reads at most 8 values past a row");
 
-// LDS written by this wave is visible to this wave's later reads: wait for its LDS operations
-// and keep the compiler from moving memory accesses across.  (Each wave of the inference kernel
-// owns its activation buffer, so no workgroup barrier is needed between layers.)
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("" ::: "memory");
+struct InferLds {
+    uint16_t *w1, *w2, *w3, *w4;  // [128][kLd1], [128][kLdH] x 2, [16][kLdH]
+    float* bias;                  // [kInferBias]
+};
+
+__device__ __forceinline__ InferLds infer_lds(uint16_t* s_mem) {
+    InferLds L;
+    L.w1 = s_mem;
+    L.w2 = L.w1 + kMlpHidden * kLd1;
+    L.w3 = L.w2 + kMlpHidden * kLdH;
+    L.w4 = L.w3 + kMlpHidden * kLdH;
+    L.bias = reinterpret_cast<float*>(s_mem + kInferLdsHalves);
+    return L;
 }
 
-// Layer 1's A operand for a wave's 16 rows from row0 (rows at or past n, and columns at or past
+__device__ __forceinline__ uint32_t pack2bf(float lo, float hi) {
+    typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
+    typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+    const f32x2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// Weights and biases into LDS, 8 values per copy.  Layer 1's row padding is zeroed (its last k
+// step multiplies it, and the first 8 values of the next row, by zero inputs).  Layers 2-4 store
+// input position p = 32 s + 8 q + t of each row from neuron 32 s + 16 (t / 4) + 4 q + t % 4: the
+// 8 positions of one copy are two runs of 4 neurons, 16 apart.
+__device__ __forceinline__ void stage_weights(const InferWeights& W, const InferLds& L, int tid, int nthreads) {
+    for (int i = tid; i < kMlpHidden * (kLd1 / 8); i += nthreads) {
+        const int r = i / (kLd1 / 8), q = i % (kLd1 / 8);
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (q < kMlpInPad / 8) v = reinterpret_cast<const uint4*>(W.w1t)[r * (kMlpInPad / 8) + q];
+        *reinterpret_cast<uint4*>(&L.w1[r * kLd1 + q * 8]) = v;
+    }
+    constexpr int kChunks = kMlpHidden / 8;
+    for (int i = tid; i < (2 * kMlpHidden + kW4Rows) * kChunks; i += nthreads) {
+        const int r = i / kChunks, p8 = i % kChunks;
+        const uint16_t* src;
+        uint16_t* dst;
+        if (r < kMlpHidden) {
+            src = W.w2t + (size_t)r * kMlpHidden;
+            dst = L.w2 + r * kLdH;
+        } else if (r < 2 * kMlpHidden) {
+            src = W.w3t + (size_t)(r - kMlpHidden) * kMlpHidden;
+            dst = L.w3 + (r - kMlpHidden) * kLdH;
+        } else {
+            src = W.w4t + (size_t)(r - 2 * kMlpHidden) * kMlpHidden;
+            dst = L.w4 + (r - 2 * kMlpHidden) * kLdH;
+        }
+        const int s = p8 >> 2, q = p8 & 3;
+        const uint2 lo = *reinterpret_cast<const uint2*>(src + 32 * s + 4 * q);
+        const uint2 hi = *reinterpret_cast<const uint2*>(src + 32 * s + 16 + 4 * q);
+        *reinterpret_cast<uint4*>(dst + 8 * p8) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+    for (int i = tid; i < kInferBias; i += nthreads)
+        L.bias[i] = i < kMlpHidden       ? W.b1[i]
+                    : i < 2 * kMlpHidden ? W.b2[i - kMlpHidden]
+                    : i < 3 * kMlpHidden ? W.b3[i - 2 * kMlpHidden]
+                                         : W.b4[i - 3 * kMlpHidden];
+}
+
+// Layer 1's B operand for a wave's 16 rows from row0 (rows at or past n, and columns at or past
 // 208, read as zeros).
 __device__ __forceinline__ void load_rows(uint4 (&a)[kKSteps1], const uint16_t* __restrict__ X, long row0, uint32_t n,
                                           int lane) {
@@ -248,100 +309,90 @@ __device__ __forceinline__ void load_rows(uint4 (&a)[kKSteps1], const uint16_t* 
     }
 }
 
-// act_out = bf16(silu(A W + b)) for a wave's 16 rows; W^T rows from LDS with stride LDW; A
-// operands from `areg` (KSTEPS registers, layer 1) or from the wave's activation buffer.
-template <int KSTEPS, int LDW, bool FROM_REGS, bool LEAN = false>
-__device__ __forceinline__ void infer_layer(const uint4* areg, const uint16_t* act_in, const uint16_t* __restrict__ wt,
-                                            const float* __restrict__ bias, uint16_t* act_out, int lane) {
-    const int r16 = lane & 15, kq = (lane >> 4) * 8;
-    // accumulators start at the bias: every element of tile j lies in column j * 16 + r16
+// hout = bf16(silu(W^T hin + b)) for a wave's 16 rows: W^T rows from LDS with stride LDW, the
+// input in registers (KSTEPS B operands), the output in the next layer's B operand order.
+template <int KSTEPS, int LDW, int GROUP = 8>
+__device__ __forceinline__ void infer_layer(const uint4 (&hin)[KSTEPS], const uint16_t* wt, const float* bias,
+                                            uint4 (&hout)[kKStepsH], int lane) {
+    const int c = lane & 15, q4 = lane >> 4;
+    // accumulators start at the bias: element r of tile j is neuron 16 j + 4 q + r
     f32x4 acc[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const float bj = bias[j * 16 + r16];
-        acc[j] = f32x4{bj, bj, bj, bj};
-    }
-    // LEAN (operands from LDS only): one k step's weights in flight at a time, leaving registers to
-    // the caller
-    constexpr int kUnrollSteps = LEAN ? 1 : KSTEPS;
-#pragma unroll kUnrollSteps
-    for (int s = 0; s < KSTEPS; ++s) {
-        const uint4 av = FROM_REGS ? areg[s] : *reinterpret_cast<const uint4*>(&act_in[r16 * kLdH + 32 * s + kq]);
+    for (int j = 0; j < 8; ++j) acc[j] = *reinterpret_cast<const f32x4*>(&bias[16 * j + 4 * q4]);
+    // the weights of one group of GROUP output tiles in flight while the previous group's
+    // products run; the scheduling barriers keep the compiler from hoisting every group's loads
+    // (and their registers) up front
+    static_assert(8 % GROUP == 0, "groups split a k step's 8 output tiles");
+    constexpr int kGroups = KSTEPS * (8 / GROUP);
+    const uint16_t* wrow = wt + c * LDW + 8 * q4;
+    uint4 w[2][GROUP];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint4 bv = *reinterpret_cast<const uint4*>(&wt[(j * 16 + r16) * LDW + 32 * s + kq]);
-            acc[j] = mfma16(av, bv, acc[j]);
+    for (int g = 0; g < GROUP; ++g) w[0][g] = *reinterpret_cast<const uint4*>(&wrow[16 * g * LDW]);
+#pragma unroll
+    for (int i = 0; i < kGroups; ++i) {
+        const int s = i / (8 / GROUP), j0 = (i % (8 / GROUP)) * GROUP;
+        if (i + 1 < kGroups) {
+            const int s1 = (i + 1) / (8 / GROUP), j1 = ((i + 1) % (8 / GROUP)) * GROUP;
+#pragma unroll
+            for (int g = 0; g < GROUP; ++g)
+                w[(i + 1) & 1][g] = *reinterpret_cast<const uint4*>(&wrow[16 * (j1 + g) * LDW + 32 * s1]);
         }
+#pragma unroll
+        for (int g = 0; g < GROUP; ++g) acc[j0 + g] = mfma16(w[i & 1][g], hin[s], acc[j0 + g]);
+        __builtin_amdgcn_sched_barrier(0);
     }
-    // every write below depends on every read above, so act_out may be act_in
+    // every write below depends on every read above, so hout may be hin
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int col = j * 16 + r16;
+    for (int s = 0; s < kKStepsH; ++s) {
+        const f32x4 a = acc[2 * s], b = acc[2 * s + 1];
+        hout[s] = make_uint4(pack2bf(silu(a[0]), silu(a[1])), pack2bf(silu(a[2]), silu(a[3])),
+                             pack2bf(silu(b[0]), silu(b[1])), pack2bf(silu(b[2]), silu(b[3])));
+    }
+}
+
+// Output layer: one 16-row A operand tile (rows 0..2 used); lanes 0..15 hold outputs 0..3 of
+// row c and write Y's 4 columns (column 3 zero).
+__device__ __forceinline__ void infer_output(const uint4 (&h)[kKStepsH], const InferLds& L, float* __restrict__ Y,
+                                             long row0, int rows_left, int lane) {
+    const int c = lane & 15, q4 = lane >> 4;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) act_out[(4 * (lane >> 4) + r) * kLdH + col] = f2bf(silu(acc[j][r]));
+    for (int s = 0; s < kKStepsH; ++s)
+        acc = mfma16(*reinterpret_cast<const uint4*>(&L.w4[c * kLdH + 32 * s + 8 * q4]), h[s], acc);
+    if (q4 == 0 && c < rows_left) {
+        const float* b4 = L.bias + 3 * kMlpHidden;
+        float* y = Y + (row0 + c) * 4;
+        y[0] = acc[0] + b4[0];
+        y[1] = acc[1] + b4[1];
+        y[2] = acc[2] + b4[2];
+        y[3] = 0.0f;
     }
 }
 
 __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(const uint16_t* __restrict__ X, InferWeights W,
                                                                  float* __restrict__ Y, uint32_t n) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_mem[kInferLdsHalves];
-    uint16_t* w1 = s_mem;                          // [128][kLd1]
-    uint16_t* w2 = w1 + kMlpHidden * kLd1;         // [128][kLdH]
-    uint16_t* w3 = w2 + kMlpHidden * kLdH;         // [128][kLdH]
+    __shared__ __attribute__((aligned(16))) uint16_t s_mem[kInferLdsHalves + 2 * kInferBias];
+    const InferLds L = infer_lds(s_mem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint16_t* act = w3 + kMlpHidden * kLdH + wave * kInferRows * kLdH;  // this wave's [16][kLdH]
 
     const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
     uint4 xa[kKSteps1];
     if (blockIdx.x < tiles) load_rows(xa, X, (long)blockIdx.x * kInferTile + wave * kInferRows, n, lane);
-
-    // weights into LDS, 8 values per copy; layer 1's row padding is zeroed (its last k step
-    // multiplies it, and the first 8 values of the next row, by zero inputs)
-    for (int i = tid; i < kMlpHidden * (kLd1 / 8); i += kInferWaves * 64) {
-        const int r = i / (kLd1 / 8), q = i % (kLd1 / 8);
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (q < kMlpInPad / 8) v = reinterpret_cast<const uint4*>(W.w1t)[r * (kMlpInPad / 8) + q];
-        *reinterpret_cast<uint4*>(&w1[r * kLd1 + q * 8]) = v;
-    }
-    for (int i = tid; i < kMlpHidden * (kMlpHidden / 8); i += kInferWaves * 64) {
-        const int r = i / (kMlpHidden / 8), q = i % (kMlpHidden / 8);
-        *reinterpret_cast<uint4*>(&w2[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w2t)[i];
-        *reinterpret_cast<uint4*>(&w3[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w3t)[i];
-    }
+    stage_weights(W, L, tid, kInferWaves * 64);
     __syncthreads();
 
-    // From here on the waves are independent: each walks the workgroup's tiles with its own
-    // activation buffer; the next tile's input rows load while the current tile is computed.
+    // From here on LDS is read-only and the waves are independent; the next tile's input rows
+    // load while the current tile is computed.
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const long row0 = (long)tile * kInferTile + wave * kInferRows;
         const int rows_left = (int)max(0L, min((long)kInferRows, (long)n - row0));
-        infer_layer<kKSteps1, kLd1, true>(xa, nullptr, w1, W.b1, act, lane);
+        uint4 h[kKStepsH];
+        infer_layer<kKSteps1, kLd1>(xa, L.w1, L.bias, h, lane);
         const uint32_t next = tile + gridDim.x;
         if (next < tiles) load_rows(xa, X, (long)next * kInferTile + wave * kInferRows, n, lane);
-        wave_lds_sync();
-        infer_layer<kMlpHidden / 32, kLdH, false>(nullptr, act, w2, W.b2, act, lane);
-        wave_lds_sync();
-        infer_layer<kMlpHidden / 32, kLdH, false>(nullptr, act, w3, W.b3, act, lane);
-        wave_lds_sync();
-        // output layer: one 16x16 tile, columns 0..2 used
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        const int r16 = lane & 15, kq = (lane >> 4) * 8;
-#pragma unroll
-        for (int s = 0; s < kMlpHidden / 32; ++s) {
-            const uint4 av = *reinterpret_cast<const uint4*>(&act[r16 * kLdH + 32 * s + kq]);
-            const uint4 bv = *reinterpret_cast<const uint4*>(&W.w4t[(long)r16 * kMlpHidden + 32 * s + kq]);
-            acc = mfma16(av, bv, acc);
-        }
-        const int col = r16;  // output column of this lane
-        if (col < 4) {
-            const float bc = col < kMlpOut ? W.b4[col] : 0.0f;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int rr = 4 * (lane >> 4) + r;
-                if (rr < rows_left) Y[(row0 + rr) * 4 + col] = col < kMlpOut ? acc[r] + bc : 0.0f;
-            }
-        }
-        wave_lds_sync();  // the next tile's first layer overwrites act
+        infer_layer<kKStepsH, kLdH>(h, L.w2, L.bias + kMlpHidden, h, lane);
+        infer_layer<kKStepsH, kLdH>(h, L.w3, L.bias + 2 * kMlpHidden, h, lane);
+        infer_output(h, L, Y, row0, rows_left, lane);
     }
 }
 
@@ -366,12 +417,6 @@ struct RowFetch {
     float3 q[kFeatSteps][2], qv[kFeatSteps][2];  // phase 2: the neighbours' records
 };
 
-__device__ __forceinline__ uint32_t pack2bf(float lo, float hi) {
-    typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
-    typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
-    const f32x2 v = {lo, hi};
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
-}
 
 __device__ __forceinline__ float3 load3(const float4* a, uint32_t i) {
     const float* f = reinterpret_cast<const float*>(a) + 4 * (size_t)i;
@@ -470,12 +515,9 @@ __device__ __forceinline__ void form_rows(uint4 (&a)[kKSteps1], const RowFetch& 
 
 __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer_closure(ClosureRows cr, InferWeights W,
                                                                          float* __restrict__ Y, uint32_t n) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_mem[kInferLdsHalves];
-    uint16_t* w1 = s_mem;
-    uint16_t* w2 = w1 + kMlpHidden * kLd1;
-    uint16_t* w3 = w2 + kMlpHidden * kLdH;
+    __shared__ __attribute__((aligned(16))) uint16_t s_mem[kInferLdsHalves + 2 * kInferBias];
+    const InferLds L = infer_lds(s_mem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint16_t* act = w3 + kMlpHidden * kLdH + wave * kInferRows * kLdH;
 
     const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
     uint4 xa[kKSteps1];
@@ -488,17 +530,7 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer_closure(ClosureR
         fetch_records(f, cr, n, lane);
         form_rows(xa, f, cr, lane);
     }
-    for (int i = tid; i < kMlpHidden * (kLd1 / 8); i += kInferWaves * 64) {
-        const int r = i / (kLd1 / 8), q = i % (kLd1 / 8);
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (q < kMlpInPad / 8) v = reinterpret_cast<const uint4*>(W.w1t)[r * (kMlpInPad / 8) + q];
-        *reinterpret_cast<uint4*>(&w1[r * kLd1 + q * 8]) = v;
-    }
-    for (int i = tid; i < kMlpHidden * (kMlpHidden / 8); i += kInferWaves * 64) {
-        const int r = i / (kMlpHidden / 8), q = i % (kMlpHidden / 8);
-        *reinterpret_cast<uint4*>(&w2[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w2t)[i];
-        *reinterpret_cast<uint4*>(&w3[r * kLdH + q * 8]) = reinterpret_cast<const uint4*>(W.w3t)[i];
-    }
+    stage_weights(W, L, tid, kInferWaves * 64);
     __syncthreads();
 
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
@@ -506,37 +538,18 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer_closure(ClosureR
         const int rows_left = (int)max(0L, min((long)kInferRows, (long)n - row0));
         const uint32_t next = tile + gridDim.x;
         const bool more = next < tiles;
-        infer_layer<kKSteps1, kLd1, true>(xa, nullptr, w1, W.b1, act, lane);
+        uint4 h[kKStepsH];
+        infer_layer<kKSteps1, kLd1, 4>(xa, L.w1, L.bias, h, lane);
         if (more) {
             fetch_entries(f, cr, (long)next * kInferTile + wave * kInferRows, n, lane);
             const uint32_t after = next + gridDim.x;
             f.listed_next = after < tiles ? row_count(cr, (long)after * kInferTile + wave * kInferRows, n, lane) : 0u;
         }
-        wave_lds_sync();
-        infer_layer<kMlpHidden / 32, kLdH, false, true>(nullptr, act, w2, W.b2, act, lane);
+        infer_layer<kKStepsH, kLdH, 4>(h, L.w2, L.bias + kMlpHidden, h, lane);
         if (more) fetch_records(f, cr, n, lane);
-        wave_lds_sync();
-        infer_layer<kMlpHidden / 32, kLdH, false, true>(nullptr, act, w3, W.b3, act, lane);
-        wave_lds_sync();
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        const int r16 = lane & 15, kq = (lane >> 4) * 8;
-#pragma unroll
-        for (int s = 0; s < kMlpHidden / 32; ++s) {
-            const uint4 av = *reinterpret_cast<const uint4*>(&act[r16 * kLdH + 32 * s + kq]);
-            const uint4 bv = *reinterpret_cast<const uint4*>(&W.w4t[(long)r16 * kMlpHidden + 32 * s + kq]);
-            acc = mfma16(av, bv, acc);
-        }
-        const int col = r16;
-        if (col < 4) {
-            const float bc = col < kMlpOut ? W.b4[col] : 0.0f;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int rr = 4 * (lane >> 4) + r;
-                if (rr < rows_left) Y[(row0 + rr) * 4 + col] = col < kMlpOut ? acc[r] + bc : 0.0f;
-            }
-        }
+        infer_layer<kKStepsH, kLdH, 4>(h, L.w3, L.bias + 2 * kMlpHidden, h, lane);
+        infer_output(h, L, Y, row0, rows_left, lane);
         if (more) form_rows(xa, f, cr, lane);
-        wave_lds_sync();
     }
 }
 
