@@ -501,9 +501,20 @@ __device__ __forceinline__ void form_rows(uint4 (&a)[kKSteps1], const RowFetch& 
             v[6 * j + 4] = keep((f.qv[s][j].y - f.pv.y) * cr.inv_v);
             v[6 * j + 5] = keep((f.qv[s][j].z - f.pv.z) * cr.inv_v);
         }
+        // c[e] = v[o + e] as a two-stage funnel shift (by 4, then by 2) over bit masks the compiler
+        // cannot see through, so that each stage is one bitfield select per value rather than a
+        // branch on the lane's offset
+        uint32_t m4 = (o & 4u) ? ~0u : 0u, m2 = (o & 2u) ? ~0u : 0u;
+        asm("" : "+v"(m4), "+v"(m2));
+        uint32_t t[10];
+#pragma unroll
+        for (int e = 0; e < 10; ++e) {
+            const uint32_t lo = __float_as_uint(v[e]), hi = e + 4 < 12 ? __float_as_uint(v[e + 4]) : 0u;
+            t[e] = (hi & m4) | (lo & ~m4);
+        }
         float c[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) c[e] = o == 0u ? v[e] : (o == 2u ? v[e + 2] : v[e + 4]);
+        for (int e = 0; e < 8; ++e) c[e] = __uint_as_float((t[e + 2] & m2) | (t[e] & ~m2));
         a[s] = make_uint4(pack2bf(c[0], c[1]), pack2bf(c[2], c[3]), pack2bf(c[4], c[5]), pack2bf(c[6], c[7]));
     }
     // columns 192..207: the density deviation, the step multiplier and the filled fraction
