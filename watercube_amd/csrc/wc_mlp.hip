@@ -442,9 +442,12 @@ __device__ __forceinline__ void fetch_entries(RowFetch& f, const ClosureRows& cr
     for (int s = 0; s < kFeatSteps; ++s) {
         const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-            f.e[s][j] = sel ? cr.sel[(size_t)f.row * kSelStride + k0 + j]
-                            : cr.nlist[(size_t)(k0 + j) * cr.stride + f.row];
+        for (int j = 0; j < 2; ++j) {
+            // one load from a selected address rather than a branch per source
+            const uint32_t* src = sel ? cr.sel + ((size_t)f.row * kSelStride + k0 + j)
+                                      : cr.nlist + ((size_t)(k0 + j) * cr.stride + f.row);
+            f.e[s][j] = *src;
+        }
     }
 }
 
@@ -461,13 +464,13 @@ __device__ __forceinline__ void fetch_records(RowFetch& f, const ClosureRows& cr
         for (int j = 0; j < 2; ++j) {
             uint32_t u = f.e[s][j];
             u = u < n ? u : f.row;
-            // neighbour k0 uses values o..5, neighbour k0 + 1 values 0..o + 1
+            // neighbour k0 uses values o..5, neighbour k0 + 1 values 0..o + 1; a half the lane does
+            // not use is read from the row's own record (already cached) instead of skipped by a
+            // branch, and never reaches the selected columns
             const bool need_pos = j == 0 ? o <= 2u : true;
             const bool need_vel = j == 0 ? true : o >= 2u;
-            f.q[s][j] = float3{0.0f, 0.0f, 0.0f};
-            f.qv[s][j] = float3{0.0f, 0.0f, 0.0f};
-            if (need_pos) f.q[s][j] = load3(cr.posrho, u);
-            if (need_vel) f.qv[s][j] = load3(cr.velp, u);
+            f.q[s][j] = load3(cr.posrho, need_pos ? u : f.row);
+            f.qv[s][j] = load3(cr.velp, need_vel ? u : f.row);
         }
     }
 }
