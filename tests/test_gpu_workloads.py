@@ -242,6 +242,39 @@ def test_closure_1m_inputs_match_numpy_on_sampled_rows(oracle):
     assert long_rows > 0
 
 
+@pytest.mark.timeout(600)
+def test_closure_1m_inference_from_lists_equals_features_then_network():
+    """The learned step's inference at 1M after 300 steps, with every kind of row in one state:
+    lists of at most 32 (all kept), of 33-40 (the nearest 32 selected in registers) and longer ones
+    (selected by the walk).  Inputs formed inside the inference kernel must give the bits of the
+    feature matrix followed by the network, on every row."""
+    p, parts, dt, _ = scenes.workload("block_1m")
+    n = p.num_particles
+    net = ClosureNet(max_batch=64).load(DEFAULT_WEIGHTS)
+    with Fluid(p, parts) as f:
+        f.set_tuning(4, 10, 1)
+        f.step(dt, 300)
+        f.sort_.run()
+        f.runDensityProg()
+        listed = f.neighbour_counts()
+        mid = int(((listed > 32) & (listed <= 40)).sum())
+        heavy = int((listed > 40).sum())
+        _log(f"closure inference 1M: {mid} rows of 33-40 listed, {heavy} longer or overflowed")
+        x = DeviceArray((n, 208), np.uint16)
+        y1 = DeviceArray((n, 4), np.float32)
+        y2 = DeviceArray((n, 4), np.float32)
+        L.check(L.lib().wc_closure_infer(f.ctx, net.h, C.c_float(4.0), y1.ptr), "wc_closure_infer")
+        L.check(L.lib().wc_closure_features(f.ctx, C.c_float(4.0), None, 0, x.ptr), "wc_closure_features")
+    net.predict(x, y2, n)
+    a, b = y1.download(), y2.download()
+    for d in (x, y1, y2):
+        d.free()
+    net.close()
+    assert mid > 0
+    bad = np.nonzero((a.view(np.uint32) != b.view(np.uint32)).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[0]} (listed {listed[bad[0]]})"
+
+
 def test_trained_closure_gains_over_plain_large_step():
     """What the committed network gains over the method without it: from states of the 1M block
     along its plain small-dt run, one step of 4 dt with the network's correction against a plain

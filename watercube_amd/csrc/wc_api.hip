@@ -142,6 +142,7 @@ struct wc_ctx {
     // learned-closure workspace (allocated on first use)
     uint16_t* cl_x = nullptr;       // features, n x 208 bf16
     float* cl_y = nullptr;          // network output, n x 4
+    uint32_t* cl_long = nullptr;    // slots whose list is longer than the selection in registers takes (n), count
     float4* cl_backup[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // two slots of (posrho, velp)
     uint32_t* cl_backup_ids[2] = {nullptr, nullptr};
     uint32_t* cl_host_rho = nullptr;  // pinned: the density maximum after each slot's learned step
@@ -246,7 +247,7 @@ void free_ctx_buffers(wc_ctx* c) {
                     c->d_tables,   b.rho_max,     b.ids[0],      b.ids[1],      c->cl_x,     c->cl_y,
                     c->cl_backup[0][0], c->cl_backup[0][1], c->cl_backup[1][0], c->cl_backup[1][1],
                     c->cl_backup_ids[0], c->cl_backup_ids[1], c->cl_rows, b.nlist, b.ncount, c->d_scene_tab,
-                    b.crowd,      b.crowd_count};
+                    b.crowd,      b.crowd_count, c->cl_long};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->cl_host_rho) (void)hipHostFree(c->cl_host_rho);
@@ -258,6 +259,7 @@ void free_ctx_buffers(wc_ctx* c) {
     c->d_scene_tab = nullptr;
     c->cl_x = nullptr;
     c->cl_y = nullptr;
+    c->cl_long = nullptr;
     c->cl_backup[0][0] = c->cl_backup[0][1] = c->cl_backup[1][0] = c->cl_backup[1][1] = nullptr;
     c->cl_backup_ids[0] = c->cl_backup_ids[1] = nullptr;
     c->cl_rows = nullptr;
@@ -1420,6 +1422,7 @@ static int closure_workspace(wc_ctx* c) {
     };
     alloc((void**)&c->cl_x, n * wc::kClosureFeatures * 2);
     alloc((void**)&c->cl_y, n * 16);
+    alloc((void**)&c->cl_long, (n + 1) * sizeof(uint32_t));
     for (int slot = 0; slot < 2; ++slot) {
         alloc((void**)&c->cl_backup[slot][0], n * sizeof(float4));
         alloc((void**)&c->cl_backup[slot][1], n * sizeof(float4));
@@ -1464,7 +1467,8 @@ WC_API int wc_closure_features(wc_ctx* c, float dt_mult, const uint32_t* rows_ho
 // selection of longer lists into cl_x's memory, then the network.
 static hipError_t closure_infer_from_lists(wc_ctx* c, wc_mlp* net, float dt_mult, float* y) {
     const uint32_t n = c->total_particles;
-    hipError_t e = wc::launch_closure_selection(c->b, n, reinterpret_cast<uint32_t*>(c->cl_x), c->stream);
+    hipError_t e = wc::launch_closure_selection(c->b, n, reinterpret_cast<uint32_t*>(c->cl_x), c->cl_long,
+                                                c->cl_long + c->particle_cap, c->stream);
     if (e != hipSuccess) return e;
     const wc::StepConsts& k = c->desc[0].c;
     const wc::ClosureRows rows{c->b.posrho[1], c->b.velp[1], c->b.nlist, c->b.ncount,

@@ -160,7 +160,11 @@ hipError_t launch_density(const DeviceBuffers& b, int variant, bool list, hipStr
 // For the single scene's particles with more than kClosureNeighbours listed neighbours (or an
 // overflowed list): the slots of the kClosureNeighbours nearest, in list order, at
 // sel[t * kClosureFeatures / 2 + k].  Other rows are left alone.
-hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t* sel, hipStream_t stream);
+// The kept neighbours' slots of every particle with more listed neighbours than the features keep
+// (or an overflowed list), at sel[t * (kClosureFeatures / 2)]; heavy (n words) and heavy_count
+// are workspace.
+hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t* sel, uint32_t* heavy,
+                                    uint32_t* heavy_count, hipStream_t stream);
 hipError_t launch_update(const DeviceBuffers& b, int variant, bool list, hipStream_t stream);
 hipError_t launch_advect(const DeviceBuffers& b, float dt, hipStream_t stream);
 // After the host changed particle buffer 2 or the parameters: recompute buffer 2's cell keys,

@@ -1058,23 +1058,34 @@ constexpr int kFeatChunk = 4;   // candidate slots loaded per step of a lane's w
 // patterns: non-negative floats order like the integers, and NaN, which the acceptance test
 // lets through, ranks above every number.
 __device__ __forceinline__ int drop_farthest(lds_ptr<uint32_t> key, lds_ptr<uint32_t> slot, int cnt, int drop) {
+    // the lane's keys into registers (all loads in flight), the `drop` largest marked one at a
+    // time (of equal keys the later entry), then one pass that closes the gaps in order -- the
+    // same entries, in the same order, as dropping and shifting one at a time
+    uint32_t kv[kFeatCap];
+#pragma unroll
+    for (int k = 0; k < kFeatCap; ++k) kv[k] = k < cnt ? key[k * kFeatBlock] : 0u;
+    uint64_t gone = 0;
     for (int m = 0; m < drop; ++m) {
         uint32_t best = 0;
         int at = 0;
-        for (int k = 0; k < cnt; ++k) {
-            const uint32_t v = key[k * kFeatBlock];
-            if (v >= best) {
-                best = v;
-                at = k;
-            }
+#pragma unroll
+        for (int k = 0; k < kFeatCap; ++k) {
+            const bool ge = k < cnt && !((gone >> k) & 1ull) && kv[k] >= best;
+            best = ge ? kv[k] : best;
+            at = ge ? k : at;
         }
-        for (int k = at; k + 1 < cnt; ++k) {
-            key[k * kFeatBlock] = key[(k + 1) * kFeatBlock];
-            slot[k * kFeatBlock] = slot[(k + 1) * kFeatBlock];
-        }
-        --cnt;
+        gone |= 1ull << at;
     }
-    return cnt;
+    int w = 0;
+#pragma unroll
+    for (int k = 0; k < kFeatCap; ++k) {
+        if (k < cnt && !((gone >> k) & 1ull)) {  // w <= k: a move never overwrites an unread entry
+            key[w * kFeatBlock] = kv[k];
+            slot[w * kFeatBlock] = slot[k * kFeatBlock];
+            ++w;
+        }
+    }
+    return w;
 }
 
 // two floats -> two bf16 (round to nearest even) in one word, lo in the low half: one
@@ -1146,6 +1157,111 @@ __device__ __forceinline__ void write_feature_row(uint4* __restrict__ row, const
     row[25] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+__global__ void k_clear_word(uint32_t* __restrict__ w) {
+    if (threadIdx.x == 0) *w = 0u;
+}
+
+// Lists a lane selects from in registers (k_closure_select) hold up to kSelRegCap or, rarely,
+// kNeighbourListMax entries; lists that overflowed go through k_closure_features on kHeavyBlocks
+// workgroups.
+constexpr int kSelRegCap = 40;
+constexpr int kHeavyBlocks = 64;
+static_assert(kSelRegCap <= (int)kNeighbourListMax, "the selection's loads stay inside the lists");
+
+// The kClosureNeighbours nearest of the m <= CAP listed neighbours of slot t, in list order, to
+// sel_out at the particle's slot: all list entries load at once, then the positions in groups of
+// 8 (the next group's gathers in flight while one is formed), and the m - kClosureNeighbours
+// farthest are dropped in registers -- the largest squared distance (by its bits, so NaN ranks
+// farthest), of equal ones the later entry.  That keeps the same neighbours as k_closure_features'
+// walk, which drops the same largest entries a few at a time (an entry dropped early is never
+// among the nearest of a larger set).
+template <int CAP>
+__device__ __forceinline__ void select_in_registers(uint32_t t, uint32_t m, const float4* __restrict__ posrho,
+                                                    const uint32_t* __restrict__ nlist, size_t stride,
+                                                    uint32_t* __restrict__ sel_out) {
+    static_assert(CAP <= (int)kNeighbourListMax, "the loads stay inside the lists");
+    // every entry of the first CAP columns (entries past m are replaced by the particle's own slot
+    // after loading)
+    uint32_t u[CAP];
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) u[k] = nlist[(size_t)k * stride + t];
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) u[k] = (uint32_t)k < m ? u[k] : t;
+    const float4 p = posrho[t];
+    constexpr int kGroup = 8;
+    static_assert(CAP % kGroup == 0, "whole groups");
+    uint32_t key[CAP];
+    float3 cur[kGroup], nxt[kGroup];
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+        const float* f = reinterpret_cast<const float*>(posrho + u[j]);
+        cur[j] = float3{f[0], f[1], f[2]};
+    }
+#pragma unroll
+    for (int g = 0; g < CAP / kGroup; ++g) {
+        if (g + 1 < CAP / kGroup) {
+#pragma unroll
+            for (int j = 0; j < kGroup; ++j) {
+                const float* f = reinterpret_cast<const float*>(posrho + u[(g + 1) * kGroup + j]);
+                nxt[j] = float3{f[0], f[1], f[2]};
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) {
+            const float rx = p.x - cur[j].x, ry = p.y - cur[j].y, rz = p.z - cur[j].z;
+            key[g * kGroup + j] = __float_as_uint((rx * rx + ry * ry) + rz * rz);
+        }
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) cur[j] = nxt[j];
+    }
+    uint64_t gone = 0;
+    for (uint32_t d = (uint32_t)kClosureNeighbours; d < m; ++d) {
+        uint32_t best = 0;
+        int at = 0;
+#pragma unroll
+        for (int k = 0; k < CAP; ++k) {
+            const bool ge = (uint32_t)k < m && !((gone >> k) & 1ull) && key[k] >= best;
+            best = ge ? key[k] : best;
+            at = ge ? k : at;
+        }
+        gone |= 1ull << at;
+    }
+    uint32_t* o = sel_out + (size_t)t * (kClosureFeatures / 2);
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < CAP; ++k)
+        if ((uint32_t)k < m && !((gone >> k) & 1ull)) o[w++] = u[k];
+}
+
+// Selection of the kept neighbours of every slot whose list is longer than kClosureNeighbours:
+// in registers for lists of up to kNeighbourListMax, while overflowed lists are appended to `heavy`
+// (one atomic per wave that has any) for k_closure_features' walk.  Rows of at most
+// kClosureNeighbours need no selection.
+__global__ __launch_bounds__(256) void k_closure_select(const float4* __restrict__ posrho, uint32_t n,
+                                                        const uint32_t* __restrict__ nlist,
+                                                        const uint32_t* __restrict__ ncount, size_t stride,
+                                                        uint32_t* __restrict__ sel_out, uint32_t* __restrict__ heavy,
+                                                        uint32_t* __restrict__ heavy_count) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t m = t < n ? ncount[t] : 0u;
+    const bool is_heavy = m > kNeighbourListMax;  // kListOverflow
+    const uint64_t hm = __ballot(is_heavy);
+    if (hm != 0) {  // wave-uniform
+        const uint32_t lane = threadIdx.x & 63;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)hm) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(heavy_count, (uint32_t)__popcll(hm));
+        base = __shfl(base, (int)leader);
+        if (is_heavy) heavy[base + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull))] = t;
+    }
+    if (m <= (uint32_t)kClosureNeighbours || is_heavy) return;
+    if (m <= (uint32_t)kSelRegCap)
+        select_in_registers<kSelRegCap>(t, m, posrho, nlist, stride, sel_out);
+    else
+        select_in_registers<(int)kNeighbourListMax>(t, m, posrho, nlist, stride, sel_out);
+}
+
 // Rows whose particle has at most kClosureNeighbours listed neighbours keep all of them, in list
 // order: written straight from the density pass's lists, with no distances and no LDS.  Other
 // rows are left to k_closure_features.
@@ -1176,21 +1292,14 @@ __global__ __launch_bounds__(256) void k_closure_features_short(Scenes S, const 
                       inv_v);
 }
 
-__global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const float4* __restrict__ posrho,
-                                                                 const float4* __restrict__ velp,
-                                                                 const uint32_t* __restrict__ skey,
-                                                                 const uint2* __restrict__ range,
-                                                                 const uint32_t* __restrict__ rows, uint32_t nrows,
-                                                                 float dt_mult, float inv_v, uint16_t* __restrict__ X,
-                                                                 const uint32_t* __restrict__ nlist,
-                                                                 const uint32_t* __restrict__ ncount,
-                                                                 bool skip_short, uint32_t* __restrict__ sel_out) {
-    static_assert(kFeatChunk <= (int)kRecordPad + 1, "chunked loads may run kFeatChunk - 1 records past a cell");
-    static_assert(kFeatCap >= kClosureNeighbours + kFeatChunk, "a chunk must fit after a selection");
-    __shared__ uint32_t s_key[kFeatCap * kFeatBlock];
-    __shared__ uint32_t s_slot[kFeatCap * kFeatBlock];
-    const uint32_t r = blockIdx.x * kFeatBlock + threadIdx.x;
-    if (r >= nrows) return;
+// One row of k_closure_features.
+__device__ __forceinline__ void closure_row(const Scenes& S, const float4* __restrict__ posrho,
+                                            const float4* __restrict__ velp, const uint32_t* __restrict__ skey,
+                                            const uint2* __restrict__ range, const uint32_t* __restrict__ rows,
+                                            uint32_t r, float dt_mult, float inv_v, uint16_t* __restrict__ X,
+                                            const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
+                                            bool skip_short, uint32_t* __restrict__ sel_out, lds_ptr<uint32_t> key,
+                                            lds_ptr<uint32_t> slot) {
     const SceneDesc& d = S.desc[0];
     const StepConsts& c = d.c;
     const uint32_t t = rows ? rows[r] : d.p_off + r;
@@ -1204,8 +1313,6 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
     if (skip_short && ncount[t] <= (uint32_t)kClosureNeighbours) return;
     const float4 p = posrho[t];
     const float4 pv = velp[t];
-    const lds_ptr<uint32_t> key = (lds_ptr<uint32_t>)s_key + threadIdx.x;
-    const lds_ptr<uint32_t> slot = (lds_ptr<uint32_t>)s_slot + threadIdx.x;
     int cnt = 0;
     auto visit = [&](uint32_t start, uint32_t end, auto own) {
         for (uint32_t u = start; u < end; u += kFeatChunk) {
@@ -1256,14 +1363,35 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
     if (cnt > kClosureNeighbours) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
 
     if (sel_out) {
-        // the kept neighbours' slots instead of the row (row r's words of X), for inference that
-        // forms the values itself
-        uint32_t* o = sel_out + (size_t)r * (kClosureFeatures / 2);
+        // the kept neighbours' slots instead of the row, at the particle's own slot, for inference
+        // that forms the values itself
+        uint32_t* o = sel_out + (size_t)(t - d.p_off) * (kClosureFeatures / 2);
         for (int k = 0; k < cnt; ++k) o[k] = slot[k * kFeatBlock];
         return;
     }
     write_feature_row(row, c, t, p, pv, cnt, [&](int k) { return slot[k * kFeatBlock]; }, posrho, velp, dt_mult,
                       inv_v);
+}
+
+__global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const float4* __restrict__ posrho,
+                                                                 const float4* __restrict__ velp,
+                                                                 const uint32_t* __restrict__ skey,
+                                                                 const uint2* __restrict__ range,
+                                                                 const uint32_t* __restrict__ rows, uint32_t nrows,
+                                                                 float dt_mult, float inv_v, uint16_t* __restrict__ X,
+                                                                 const uint32_t* __restrict__ nlist,
+                                                                 const uint32_t* __restrict__ ncount,
+                                                                 bool skip_short, uint32_t* __restrict__ sel_out,
+                                                                 const uint32_t* __restrict__ nrows_dev) {
+    static_assert(kFeatChunk <= (int)kRecordPad + 1, "chunked loads may run kFeatChunk - 1 records past a cell");
+    static_assert(kFeatCap >= kClosureNeighbours + kFeatChunk, "a chunk must fit after a selection");
+    __shared__ uint32_t s_key[kFeatCap * kFeatBlock];
+    __shared__ uint32_t s_slot[kFeatCap * kFeatBlock];
+    if (nrows_dev) nrows = *nrows_dev;  // a count an earlier kernel wrote
+    // rows r, r + the grid, ...: one row per lane when the grid covers them all
+    for (uint32_t r = blockIdx.x * kFeatBlock + threadIdx.x; r < nrows; r += gridDim.x * kFeatBlock)
+        closure_row(S, posrho, velp, skey, range, rows, r, dt_mult, inv_v, X, nlist, ncount, skip_short, sel_out,
+                    (lds_ptr<uint32_t>)s_key + threadIdx.x, (lds_ptr<uint32_t>)s_slot + threadIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1457,15 +1585,25 @@ hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows,
     }
     hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(nrows, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc,
                        b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, 1.0f / v_ref, X,
-                       lists ? b.nlist : nullptr, lists ? b.ncount : nullptr, lists, nullptr);
+                       lists ? b.nlist : nullptr, lists ? b.ncount : nullptr, lists, nullptr, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t* sel, hipStream_t s) {
+hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t* sel, uint32_t* heavy,
+                                    uint32_t* heavy_count, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (!b.nlist || !b.ncount || !sel) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(n, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc, b.posrho[1],
-                       b.velp[1], b.skey, b.range, nullptr, n, 0.0f, 1.0f, nullptr, b.nlist, b.ncount, true, sel);
+    if (!b.nlist || !b.ncount || !sel || !heavy || !heavy_count) return hipErrorInvalidValue;
+    // a kernel clears the count (no memset node in a captured graph); the register selection runs
+    // on every slot and lists the rare longer rows, which the walk then takes on a small grid
+    hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, s, heavy_count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_closure_select, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.posrho[1], n, b.nlist, b.ncount,
+                       (size_t)b.sc.total_particles, sel, heavy, heavy_count);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_closure_features, dim3(kHeavyBlocks), dim3(kFeatBlock), 0, s, b.sc, b.posrho[1], b.velp[1],
+                       b.skey, b.range, heavy, n, 0.0f, 1.0f, nullptr, b.nlist, b.ncount, true, sel, heavy_count);
     return hipGetLastError();
 }
 
