@@ -190,7 +190,7 @@ __global__ void k_cast(const float* __restrict__ w, uint16_t* __restrict__ rowma
 // ------------------------------------------------------------------------------------------
 // Fused inference: Y[n x 4] = MLP(X[n x 208]).  Persistent: one workgroup of eight waves per CU
 // copies all four layers' weights and biases into LDS once (bf16, transposed [out][in], each row
-// padded by 8 values so that the 16 rows an MFMA operand read touches fall in different banks)
+// padded so that the 16 rows an MFMA operand read touches spread over the banks)
 // and then walks tiles of 128 rows; each wave takes 16 rows of a tile through the four layers on
 // v_mfma_f32_16x16x32_bf16.
 //
@@ -220,7 +220,10 @@ __device__ __forceinline__ f32x4 mfma16(uint4 a, uint4 b, f32x4 c) {
 }
 
 constexpr int kLd1 = kMlpInPad + 8;   // LDS row length of layer 1's weights (208 used, 8 zero)
-constexpr int kLdH = kMlpHidden + 8;  // LDS row length of layers 2-4's weights
+// LDS row length of layers 2-4's weights: 72 dwords.  Measured on gfx950 (SQ_LDS_BANK_CONFLICT
+// over the kernel's operand reads): rows of 68, 76, 80, 84 or 92 dwords spend 45 % of the LDS's
+// active cycles in bank conflicts, 96 dwords 63 %, and 72 or 88 dwords 27 %.
+constexpr int kLdH = kMlpHidden + 16;
 constexpr int kInferWaves = 8;
 constexpr int kInferRows = 16;                         // rows per wave and tile
 constexpr int kInferTile = kInferWaves * kInferRows;   // rows per tile
