@@ -84,9 +84,9 @@ def test_training_fits_a_teacher_network():
 def test_timed_training_steps_are_training_steps():
     """wc_mlp_profile_train runs one warm-up step and `reps` timed steps of the same kernels as
     wc_mlp_train_step: its parameters afterwards equal those of reps + 1 plain training steps.  A
-    batch under 4,096 rows keeps every GEMM unsplit, so each gradient is summed in one order."""
+    batch under 1,024 rows keeps every GEMM unsplit, so each gradient is summed in one order."""
     rng = np.random.default_rng(2)
-    m = 2048
+    m = 1000
     a = ClosureNet(max_batch=m, seed=5)
     b = ClosureNet(max_batch=m, seed=5)
     xb, t = _random_batch(rng, m)

@@ -1247,6 +1247,7 @@ WC_API int wc_mlp_create(uint32_t max_batch, uint64_t seed, wc_mlp** out) {
     alloc((void**)&m.gbuf, B * wc::kMlpHidden * 4);
     alloc((void**)&m.d, B * wc::kMlpHidden * 2);
     alloc((void**)&m.loss, 64);
+    alloc((void**)&m.colpart, (B + wc::kColsumRows - 1) / wc::kColsumRows * wc::kMlpHidden * 4);
     if (!ok) {
         wc_mlp_destroy(h);
         return fail(WC_ERR_NO_MEMORY, "network buffers for a batch of %u", max_batch);
@@ -1290,7 +1291,8 @@ WC_API int wc_mlp_destroy(wc_mlp* h) {
     wc::Mlp& m = h->m;
     if (m.stream) (void)hipStreamSynchronize(m.stream);
     void* ptrs[] = {m.master, m.grad, m.adam_m, m.adam_v, m.w1, m.w1t, m.w2, m.w2t, m.w3, m.w3t, m.w4, m.w4t,
-                    m.z[0], m.z[1], m.z[2], m.h[0], m.h[1], m.h[2], m.yacc, m.d4, m.gbuf, m.d, m.loss};
+                    m.z[0], m.z[1], m.z[2], m.h[0], m.h[1], m.h[2], m.yacc, m.d4, m.gbuf, m.d, m.loss,
+                    m.colpart};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m.stream) (void)hipStreamDestroy(m.stream);

@@ -46,36 +46,97 @@ __device__ __forceinline__ f32x16 mfma(uint4 a, uint4 b, f32x16 c) {
 // ------------------------------------------------------------------------------------------
 // Generic GEMM: C[m][n] (+)= sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) =
 // B[k*sbk + n*sbn] (bf16), C fp32 row-major with leading dimension ldc.  64x64 tile per
-// workgroup (2x2 waves of 32x32), K in steps of 16, split over gridDim.z with atomic adds.
+// workgroup (2x2 waves of 32x32 on v_mfma_f32_32x32x16_bf16); K in stages of 64 staged through
+// LDS as [row][k] (k contiguous, rows padded by 8), the next stage's global loads in flight while
+// the current one is multiplied; split over gridDim.z with atomic adds.
+//
+// Each operand tile (64 rows x 64 k) loads as 512 runs of 8 values along the operand's contiguous
+// dimension -- k, or the row (m for A, n for B) when ROW_CONTIG -- two runs per thread, as one
+// 16-byte load where the run lies inside the matrix and the addresses are aligned (`vec`), value
+// by value (zeros outside) elsewhere.  Row-contiguous runs are transposed as they go into LDS.
 // ------------------------------------------------------------------------------------------
+constexpr int kGemmTile = 64;
+constexpr int kGemmKStage = 64;
+constexpr int kGemmLdk = kGemmKStage + 8;
+
+template <bool ROW_CONTIG>
+__device__ __forceinline__ void gemm_load(uint4 (&v)[2], const uint16_t* __restrict__ X, long s_row, long s_k, int row0,
+                                          int nrows, int k0, int kend, bool vec, int tid) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int q = tid * 2 + i;
+        const int a = q >> 3, b = (q & 7) * 8;
+        const int gr = row0 + (ROW_CONTIG ? b : a), gk = k0 + (ROW_CONTIG ? a : b);
+        const bool full = ROW_CONTIG ? (gr + 8 <= nrows && gk < kend) : (gr < nrows && gk + 8 <= kend);
+        if (vec && full) {
+            v[i] = *reinterpret_cast<const uint4*>(&X[(long)gr * s_row + (long)gk * s_k]);
+        } else {
+            uint32_t w[4];
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                uint32_t e[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int rr = ROW_CONTIG ? gr + j + h : gr, kk = ROW_CONTIG ? gk : gk + j + h;
+                    e[h] = (rr < nrows && kk < kend) ? X[(long)rr * s_row + (long)kk * s_k] : 0u;
+                }
+                w[j / 2] = e[0] | (e[1] << 16);
+            }
+            v[i] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+template <bool ROW_CONTIG>
+__device__ __forceinline__ void gemm_store(const uint4 (&v)[2], uint16_t* tile, int tid) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int q = tid * 2 + i;
+        const int a = q >> 3, b = (q & 7) * 8;
+        if (!ROW_CONTIG) {
+            *reinterpret_cast<uint4*>(&tile[a * kGemmLdk + b]) = v[i];
+        } else {
+            const uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) tile[(b + j) * kGemmLdk + a] = (uint16_t)(w[j / 2] >> (16 * (j & 1)));
+        }
+    }
+}
+
+template <bool A_ROWS, bool B_ROWS>
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, long sam, long sak,
                                               const uint16_t* __restrict__ B, long sbk, long sbn,
                                               float* __restrict__ C, long ldc, int M, int N, int K, int k_split,
-                                              int atomic) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[64 * 16];  // [m][k]
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[64 * 16];  // [n][k]
+                                              int atomic, int vec_a, int vec_b) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[kGemmTile * kGemmLdk];  // [m][k]
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[kGemmTile * kGemmLdk];  // [n][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int m0 = blockIdx.y * kGemmTile, n0 = blockIdx.x * kGemmTile;
     const int kbeg = blockIdx.z * k_split;
     const int kend = min(K, kbeg + k_split);
     f32x16 acc = {};
-    for (int k0 = kbeg; k0 < kend; k0 += 16) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = tid * 4 + q;
-            const int r = e >> 4, kk = e & 15;
-            const int gk = k0 + kk;
-            const int gm = m0 + r, gn = n0 + r;
-            As[e] = (gm < M && gk < kend) ? A[(long)gm * sam + (long)gk * sak] : (uint16_t)0;
-            Bs[e] = (gn < N && gk < kend) ? B[(long)gk * sbk + (long)gn * sbn] : (uint16_t)0;
+    uint4 ra[2], rb[2];
+    if (kbeg < kend) {
+        gemm_load<A_ROWS>(ra, A, sam, sak, m0, M, kbeg, kend, vec_a != 0, tid);
+        gemm_load<B_ROWS>(rb, B, sbn, sbk, n0, N, kbeg, kend, vec_b != 0, tid);
+    }
+    for (int k0 = kbeg; k0 < kend; k0 += kGemmKStage) {
+        __syncthreads();  // the previous stage's operand reads are done
+        gemm_store<A_ROWS>(ra, As, tid);
+        gemm_store<B_ROWS>(rb, Bs, tid);
+        __syncthreads();
+        if (k0 + kGemmKStage < kend) {
+            gemm_load<A_ROWS>(ra, A, sam, sak, m0, M, k0 + kGemmKStage, kend, vec_a != 0, tid);
+            gemm_load<B_ROWS>(rb, B, sbn, sbk, n0, N, k0 + kGemmKStage, kend, vec_b != 0, tid);
         }
-        __syncthreads();
-        const int kq = (lane >> 5) * 8;
-        const uint4 a = *reinterpret_cast<const uint4*>(&As[(wm * 32 + (lane & 31)) * 16 + kq]);
-        const uint4 b = *reinterpret_cast<const uint4*>(&Bs[(wn * 32 + (lane & 31)) * 16 + kq]);
-        acc = mfma(a, b, acc);
-        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kGemmKStage / 16; ++ks) {
+            const int kq = ks * 16 + (lane >> 5) * 8;
+            const uint4 a = *reinterpret_cast<const uint4*>(&As[(wm * 32 + (lane & 31)) * kGemmLdk + kq]);
+            const uint4 b = *reinterpret_cast<const uint4*>(&Bs[(wn * 32 + (lane & 31)) * kGemmLdk + kq]);
+            acc = mfma(a, b, acc);
+        }
     }
     const int col = n0 + wn * 32 + (lane & 31);
 #pragma unroll
@@ -148,19 +209,33 @@ __global__ void k_silu_backward(const float* __restrict__ G, const float* __rest
 }
 
 // out[c] += sum over rows of D[m][c] (bias gradients), D bf16 with ld columns
-__global__ void k_colsum(const uint16_t* __restrict__ D, float* __restrict__ out, int M, int n, int ld) {
-    const int c = blockIdx.x;
+// Bias gradients: out[c] += sum over the M rows of D[m][c], c < n.  Partial sums per chunk of
+// kColsumRows rows (64 columns x 4 row groups per workgroup, coalesced along the row), then one
+// pass over the chunks; every sum is taken in a fixed order, so the result does not depend on
+// scheduling.
+__global__ __launch_bounds__(256) void k_colsum_part(const uint16_t* __restrict__ D, float* __restrict__ part, int M,
+                                                     int n, int ld) {
+    const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + l, r0 = blockIdx.y * kColsumRows;
+    const int r1 = min(M, r0 + kColsumRows);
     float s = 0.0f;
-    for (int m = threadIdx.x; m < M; m += blockDim.x) s += bf2f(D[(long)m * ld + c]);
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
-    __shared__ float part[16];
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.0f;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
-        if (c < n) out[c] += t;
+    if (c < n) {
+#pragma unroll 8
+        for (int r = r0 + g; r < r1; r += 4) s += bf2f(D[(long)r * ld + c]);
     }
+    __shared__ float red[4][64];
+    red[g][l] = s;
+    __syncthreads();
+    if (g == 0 && c < n) part[(long)blockIdx.y * n + c] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+}
+
+__global__ void k_colsum_sum(const float* __restrict__ part, float* __restrict__ out, int chunks, int n) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    float s = 0.0f;
+#pragma unroll 8
+    for (int i = 0; i < chunks; ++i) s += part[(long)i * n + c];
+    out[c] += s;
 }
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -580,20 +655,34 @@ inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) / per); 
 hipError_t gemm(const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, float* C, long ldc, int M,
                 int N, int K, bool accumulate, hipStream_t s) {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-    // split K so that long reductions (weight gradients over a batch) spread over the chip
+    // split K so that long reductions (weight gradients over a batch) spread over the chip: at
+    // least 256 k per split, up to about 512 workgroups
+    const long tiles = (long)((M + kGemmTile - 1) / kGemmTile) * ((N + kGemmTile - 1) / kGemmTile);
     int splits = 1;
-    const long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
-    if (K >= 4096 && tiles < 256) splits = (int)std::min<long>((K + 2047) / 2048, 512 / tiles + 1);
+    if (K >= 1024 && tiles < 256) splits = (int)std::min<long>((K + 255) / 256, (512 + tiles - 1) / tiles);
     int k_split = (K + splits - 1) / splits;
-    k_split = (k_split + 15) / 16 * 16;
+    k_split = (k_split + kGemmKStage - 1) / kGemmKStage * kGemmKStage;
     splits = (K + k_split - 1) / k_split;
     const bool atomic = accumulate || splits > 1;
     if (atomic && !accumulate) {
         hipError_t e = hipMemset2DAsync(C, ldc * sizeof(float), 0, N * sizeof(float), M, s);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_gemm, dim3((N + 63) / 64, (M + 63) / 64, splits), dim3(256), 0, s, A, sam, sak, B, sbk, sbn,
-                       C, ldc, M, N, K, k_split, atomic ? 1 : 0);
+    // which dimension of each operand is contiguous, and whether its runs of 8 may load as one
+    // 16-byte access (aligned base, the other stride a multiple of 8)
+    const bool a_rows = sam == 1, b_rows = sbn == 1;
+    auto aligned = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+    const int vec_a = aligned(A) && (a_rows ? sak % 8 == 0 : (sak == 1 && sam % 8 == 0));
+    const int vec_b = aligned(B) && (b_rows ? sbk % 8 == 0 : (sbk == 1 && sbn % 8 == 0));
+    const dim3 grid((N + kGemmTile - 1) / kGemmTile, (M + kGemmTile - 1) / kGemmTile, splits);
+#define WC_GEMM(AR, BR)                                                                                            \
+    hipLaunchKernelGGL((k_gemm<AR, BR>), grid, dim3(256), 0, s, A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, k_split, \
+                       atomic ? 1 : 0, vec_a, vec_b)
+    if (a_rows && b_rows) WC_GEMM(true, true);
+    else if (a_rows) WC_GEMM(true, false);
+    else if (b_rows) WC_GEMM(false, true);
+    else WC_GEMM(false, false);
+#undef WC_GEMM
     return hipGetLastError();
 }
 
@@ -684,6 +773,16 @@ hipError_t mlp_forward_train(Mlp& m, const uint16_t* X, const float* T, float* Y
     return hipSuccess;
 }
 
+// out[c] += column sums of the M x n bf16 matrix D (leading dimension ld).
+static hipError_t colsum(Mlp& m, const uint16_t* D, float* out, int M, int n, int ld, hipStream_t s) {
+    const int chunks = (M + kColsumRows - 1) / kColsumRows;
+    hipLaunchKernelGGL(k_colsum_part, dim3((n + 63) / 64, chunks), dim3(256), 0, s, D, m.colpart, M, n, ld);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_colsum_sum, dim3((n + 127) / 128), dim3(128), 0, s, m.colpart, out, chunks, n);
+    return hipGetLastError();
+}
+
 // Gradients of the mean squared error into m.grad (after mlp_forward_train with targets).
 hipError_t mlp_backward(Mlp& m, const uint16_t* X, uint32_t M, hipStream_t s) {
     const MlpLayout L = mlp_layout();
@@ -692,8 +791,7 @@ hipError_t mlp_backward(Mlp& m, const uint16_t* X, uint32_t M, hipStream_t s) {
     WC_TRY(hipMemsetAsync(g, 0, L.total * sizeof(float), s));
     // layer 4: dW4 = H3^T D4, db4 = colsum D4
     WC_TRY(gemm(m.h[2], 1, kMlpHidden, m.d4, kMlpOutPad, 1, g + L.w4, kMlpOutPad, kMlpHidden, kMlpOutPad, M, true, s));
-    hipLaunchKernelGGL(k_colsum, dim3(kMlpOutPad), dim3(256), 0, s, m.d4, g + L.b4, (int)M, kMlpOut, kMlpOutPad);
-    WC_TRY(hipGetLastError());
+    WC_TRY(colsum(m, m.d4, g + L.b4, (int)M, kMlpOut, kMlpOutPad, s));
     // G3 = D4 W4^T, D3 = G3 * silu'(Z3)
     WC_TRY(gemm(m.d4, kMlpOutPad, 1, m.w4, 1, kMlpOutPad, m.gbuf, kMlpHidden, M, kMlpHidden, kMlpOutPad, false, s));
     hipLaunchKernelGGL(k_silu_backward, dim3(nblk((long)M * kMlpHidden, 256)), dim3(256), 0, s, m.gbuf, m.z[2], m.d,
@@ -706,9 +804,7 @@ hipError_t mlp_backward(Mlp& m, const uint16_t* X, uint32_t M, hipStream_t s) {
         const uint16_t* Hin = l == 0 ? X : m.h[l - 1];
         const int kin = l == 0 ? kMlpInPad : kMlpHidden;
         WC_TRY(gemm(Hin, 1, kin, m.d, kMlpHidden, 1, g + wofs[l], kMlpHidden, kin, kMlpHidden, M, true, s));
-        hipLaunchKernelGGL(k_colsum, dim3(kMlpHidden), dim3(256), 0, s, m.d, g + bofs[l], (int)M, kMlpHidden,
-                           kMlpHidden);
-        WC_TRY(hipGetLastError());
+        WC_TRY(colsum(m, m.d, g + bofs[l], (int)M, kMlpHidden, kMlpHidden, s));
         if (l == 0) break;
         // G_{l-1} = D_l W_l^T, D_{l-1} = G * silu'(Z_{l-1})
         WC_TRY(gemm(m.d, kMlpHidden, 1, wrm[l], 1, kMlpHidden, m.gbuf, kMlpHidden, M, kMlpHidden, kMlpHidden, false,

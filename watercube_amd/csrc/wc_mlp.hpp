@@ -21,6 +21,8 @@ struct MlpLayout {
 };
 MlpLayout mlp_layout();
 
+constexpr int kColsumRows = 1024;  // rows per partial column sum of the bias gradients
+
 struct Mlp {
     uint32_t max_batch = 0;
     float* master = nullptr;  // fp32 parameters (MlpLayout)
@@ -38,6 +40,7 @@ struct Mlp {
     float* gbuf = nullptr;                         // back-propagated gradient (fp32, 128 columns)
     uint16_t* d = nullptr;                         // hidden delta (bf16, 128 columns)
     float* loss = nullptr;                         // sum of squared errors of the last batch
+    float* colpart = nullptr;                      // column sums per chunk of kColsumRows rows
     hipStream_t stream = nullptr;
     int device = 0;
 };
