@@ -333,15 +333,26 @@ __device__ __forceinline__ uint32_t pack2bf(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
-// Weights and biases into LDS, 8 values per copy.  Layer 1's row padding is zeroed (its last k
-// step multiplies it, and the first 8 values of the next row, by zero inputs).  Layers 2-4 store
-// input position p = 32 s + 8 q + t of each row from neuron 32 s + 16 (t / 4) + 4 q + t % 4: the
-// 8 positions of one copy are two runs of 4 neurons, 16 apart.
+// Layer 1's input order.  Its first six k steps (192 positions) hold the 32 neighbours' values:
+// lane quarter q takes neighbours 8q .. 8q + 7, whose 48 values are features 48q .. 48q + 47, as
+// 8 consecutive features per k step: position p = 32 s + 8 q + t holds feature 48 q + 8 s + t.
+// Every lane then starts step s at the same offset, (8 s) mod 6, into its first neighbour of the
+// step, a constant.  Step 6 holds features 192 .. 223 in order (the three scalars, then zeros).
+__device__ __forceinline__ int layer1_chunk(int p8) {  // source chunk of 8 features for chunk p8
+    return p8 < 24 ? 6 * (p8 & 3) + (p8 >> 2) : p8;
+}
+
+// Weights and biases into LDS, 8 values per copy.  Layer 1's input columns are stored in the
+// order above, and its row padding is zeroed (its last k step multiplies it, and the first 8
+// values of the next row, by zero inputs).  Layers 2-4 store input position p = 32 s + 8 q + t of
+// each row from neuron 32 s + 16 (t / 4) + 4 q + t % 4: the 8 positions of one copy are two runs
+// of 4 neurons, 16 apart.
 __device__ __forceinline__ void stage_weights(const InferWeights& W, const InferLds& L, int tid, int nthreads) {
     for (int i = tid; i < kMlpHidden * (kLd1 / 8); i += nthreads) {
         const int r = i / (kLd1 / 8), q = i % (kLd1 / 8);
+        const int src = layer1_chunk(q);
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (q < kMlpInPad / 8) v = reinterpret_cast<const uint4*>(W.w1t)[r * (kMlpInPad / 8) + q];
+        if (src < kMlpInPad / 8) v = reinterpret_cast<const uint4*>(W.w1t)[r * (kMlpInPad / 8) + src];
         *reinterpret_cast<uint4*>(&L.w1[r * kLd1 + q * 8]) = v;
     }
     constexpr int kChunks = kMlpHidden / 8;
@@ -371,19 +382,21 @@ __device__ __forceinline__ void stage_weights(const InferWeights& W, const Infer
                                          : W.b4[i - 3 * kMlpHidden];
 }
 
-// Layer 1's B operand for a wave's 16 rows from row0 (rows at or past n, and columns at or past
-// 208, read as zeros).
+// Layer 1's B operand for a wave's 16 rows from row0, in layer 1's input order (rows at or past
+// n, and features at or past 208, read as zeros).
 __device__ __forceinline__ void load_rows(uint4 (&a)[kKSteps1], const uint16_t* __restrict__ X, long row0, uint32_t n,
                                           int lane) {
-    const int row = lane & 15, kq = (lane >> 4) * 8;
+    const int row = lane & 15, q4 = lane >> 4;
     const bool live = row0 + row < (long)n;
 #pragma unroll
     for (int s = 0; s < kKSteps1; ++s) a[s] = make_uint4(0u, 0u, 0u, 0u);
     if (live) {
-        const uint16_t* src = X + (row0 + row) * kMlpInPad + kq;
+        const uint16_t* src = X + (row0 + row) * kMlpInPad;
 #pragma unroll
-        for (int s = 0; s < kKSteps1; ++s)
-            if (32 * s + kq < kMlpInPad) a[s] = *reinterpret_cast<const uint4*>(src + 32 * s);
+        for (int s = 0; s < kKSteps1; ++s) {
+            const int chunk = layer1_chunk(4 * s + q4);
+            if (chunk < kMlpInPad / 8) a[s] = *reinterpret_cast<const uint4*>(src + 8 * chunk);
+        }
     }
 }
 
@@ -476,14 +489,15 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(const uint16_t* 
 
 // ------------------------------------------------------------------------------------------
 // Inference on the closure's inputs without the feature matrix: each lane forms its own layer-1
-// operand from the neighbour lists.  Lane l (row l % 16, quarter q = l / 16) holds columns
-// 32 s + 8 q .. + 7 of k-step s; for s < 6 these are values o..5 of neighbour
-// k0 = (32 s + 8 q) / 6, where o = 32 s + 8 q - 6 k0 is 0, 2 or 4, then values 0..o + 1 of
-// neighbour k0 + 1: two neighbours per step.  The next tile's rows are fetched in phases between
-// the current tile's layers (the row counts a tile earlier still, then the list entries, the
-// records, and last the values), so the loads land while the matrix cores work.
+// operand from the neighbour lists.  Lane l (row l % 16, quarter q = l / 16) holds, in layer 1's
+// input order, the values of neighbours 8q .. 8q + 7 of its row: k step s < 6 takes neighbours
+// a_s = floor(8 s / 6) and a_s + 1 of the eight, from value (8 s) mod 6 of the first.  The next
+// tile's rows are fetched in phases between the current tile's layers (the row counts a tile
+// earlier still, then the list entries, the records, and last the values), so the loads land
+// while the matrix cores work.
 // ------------------------------------------------------------------------------------------
 constexpr int kFeatSteps = 6;  // k-steps of neighbour values (192 columns)
+constexpr int kLaneNeighbours = 8;
 
 struct RowFetch {
     uint32_t row, listed;
@@ -491,10 +505,9 @@ struct RowFetch {
     uint32_t listed_next;  // the count of the row one tile further on, loaded a tile early
     float4 p;              // own position and density
     float3 pv;             // own velocity
-    uint32_t e[kFeatSteps][2];                  // phase 1: list entries (or the selection's slots)
-    float3 q[kFeatSteps][2], qv[kFeatSteps][2];  // phase 2: the neighbours' records
+    uint32_t e[kLaneNeighbours];                          // phase 1: list entries (or selected slots)
+    float3 q[kLaneNeighbours], qv[kLaneNeighbours];       // phase 2: the neighbours' records
 };
-
 
 __device__ __forceinline__ float3 load3(const float4* a, uint32_t i) {
     const float* f = reinterpret_cast<const float*>(a) + 4 * (size_t)i;
@@ -517,39 +530,22 @@ __device__ __forceinline__ void fetch_entries(RowFetch& f, const ClosureRows& cr
     f.pv = load3(cr.velp, f.row);
     const bool sel = f.listed > 32u;
 #pragma unroll
-    for (int s = 0; s < kFeatSteps; ++s) {
-        const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            // one load from a selected address rather than a branch per source
-            const uint32_t* src = sel ? cr.sel + ((size_t)f.row * kSelStride + k0 + j)
-                                      : cr.nlist + ((size_t)(k0 + j) * cr.stride + f.row);
-            f.e[s][j] = *src;
-        }
+    for (int i = 0; i < kLaneNeighbours; ++i) {
+        const uint32_t k = (uint32_t)(kLaneNeighbours * q4 + i);
+        // one load from a selected address rather than a branch per source
+        const uint32_t* src = sel ? cr.sel + ((size_t)f.row * kSelStride + k) : cr.nlist + ((size_t)k * cr.stride + f.row);
+        f.e[i] = *src;
     }
 }
 
-// phase 2: the neighbours' records, only the halves the lane's columns use (entries past a list
-// may be stale: a slot out of range is replaced by the row's own, so every load stays inside the
-// buffers)
+// phase 2: the neighbours' records (entries past a list may be stale: a slot out of range is
+// replaced by the row's own, so every load stays inside the buffers)
 __device__ __forceinline__ void fetch_records(RowFetch& f, const ClosureRows& cr, uint32_t n, int lane) {
-    const int q4 = lane >> 4;
 #pragma unroll
-    for (int s = 0; s < kFeatSteps; ++s) {
-        const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
-        const uint32_t o = (uint32_t)(32 * s + 8 * q4) - 6u * k0;  // 0, 2 or 4
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            uint32_t u = f.e[s][j];
-            u = u < n ? u : f.row;
-            // neighbour k0 uses values o..5, neighbour k0 + 1 values 0..o + 1; a half the lane does
-            // not use is read from the row's own record (already cached) instead of skipped by a
-            // branch, and never reaches the selected columns
-            const bool need_pos = j == 0 ? o <= 2u : true;
-            const bool need_vel = j == 0 ? true : o >= 2u;
-            f.q[s][j] = load3(cr.posrho, need_pos ? u : f.row);
-            f.qv[s][j] = load3(cr.velp, need_vel ? u : f.row);
-        }
+    for (int i = 0; i < kLaneNeighbours; ++i) {
+        const uint32_t u = f.e[i] < n ? f.e[i] : f.row;
+        f.q[i] = load3(cr.posrho, u);
+        f.qv[i] = load3(cr.velp, u);
     }
 }
 
@@ -564,38 +560,26 @@ __device__ __forceinline__ void form_rows(uint4 (&a)[kKSteps1], const RowFetch& 
     }
     const uint32_t cnt = f.listed > 32u ? 32u : f.listed;  // longer lists keep their 32 nearest
     const float inv_h = 1.0f / cr.h;
+    float v[kLaneNeighbours][6];
+#pragma unroll
+    for (int i = 0; i < kLaneNeighbours; ++i) {
+        // zeros past the list by masking the bits (an opaque mask keeps the loads unconditional)
+        uint32_t m = (uint32_t)(kLaneNeighbours * q4 + i) < cnt ? ~0u : 0u;
+        asm("" : "+v"(m));
+        auto keep = [m](float x) { return __uint_as_float(__float_as_uint(x) & m); };
+        v[i][0] = keep((f.q[i].x - f.p.x) * inv_h);
+        v[i][1] = keep((f.q[i].y - f.p.y) * inv_h);
+        v[i][2] = keep((f.q[i].z - f.p.z) * inv_h);
+        v[i][3] = keep((f.qv[i].x - f.pv.x) * cr.inv_v);
+        v[i][4] = keep((f.qv[i].y - f.pv.y) * cr.inv_v);
+        v[i][5] = keep((f.qv[i].z - f.pv.z) * cr.inv_v);
+    }
 #pragma unroll
     for (int s = 0; s < kFeatSteps; ++s) {
-        const uint32_t k0 = (uint32_t)(32 * s + 8 * q4) / 6u;
-        const uint32_t o = (uint32_t)(32 * s + 8 * q4) - 6u * k0;  // 0, 2 or 4
-        float v[12];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            // zeros past the list by masking the bits (an opaque mask keeps the loads unconditional)
-            uint32_t m = k0 + (uint32_t)j < cnt ? ~0u : 0u;
-            asm("" : "+v"(m));
-            auto keep = [m](float x) { return __uint_as_float(__float_as_uint(x) & m); };
-            v[6 * j + 0] = keep((f.q[s][j].x - f.p.x) * inv_h);
-            v[6 * j + 1] = keep((f.q[s][j].y - f.p.y) * inv_h);
-            v[6 * j + 2] = keep((f.q[s][j].z - f.p.z) * inv_h);
-            v[6 * j + 3] = keep((f.qv[s][j].x - f.pv.x) * cr.inv_v);
-            v[6 * j + 4] = keep((f.qv[s][j].y - f.pv.y) * cr.inv_v);
-            v[6 * j + 5] = keep((f.qv[s][j].z - f.pv.z) * cr.inv_v);
-        }
-        // c[e] = v[o + e] as a two-stage funnel shift (by 4, then by 2) over bit masks the compiler
-        // cannot see through, so that each stage is one bitfield select per value rather than a
-        // branch on the lane's offset
-        uint32_t m4 = (o & 4u) ? ~0u : 0u, m2 = (o & 2u) ? ~0u : 0u;
-        asm("" : "+v"(m4), "+v"(m2));
-        uint32_t t[10];
-#pragma unroll
-        for (int e = 0; e < 10; ++e) {
-            const uint32_t lo = __float_as_uint(v[e]), hi = e + 4 < 12 ? __float_as_uint(v[e + 4]) : 0u;
-            t[e] = (hi & m4) | (lo & ~m4);
-        }
+        const int first = (8 * s) / 6, o = (8 * s) % 6;  // compile-time
         float c[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) c[e] = __uint_as_float((t[e + 2] & m2) | (t[e] & ~m2));
+        for (int e = 0; e < 8; ++e) c[e] = o + e < 6 ? v[first][o + e] : v[first + 1][o + e - 6];
         a[s] = make_uint4(pack2bf(c[0], c[1]), pack2bf(c[2], c[3]), pack2bf(c[4], c[5]), pack2bf(c[6], c[7]));
     }
     // columns 192..207: the density deviation, the step multiplier and the filled fraction
