@@ -551,6 +551,7 @@ def main(argv=None) -> int:
     a.add_argument("--epochs", type=int, default=8)
     a.add_argument("--batch", type=int, default=8192)
     a.add_argument("--lr", type=float, default=2e-3)
+    a.add_argument("--ks", default="2,4", help="step multipliers K of the training targets, comma-separated")
     a.add_argument("--log", default=None)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
@@ -567,13 +568,15 @@ def main(argv=None) -> int:
 
     if args.cmd == "train":
         t0 = time.perf_counter()
-        x, t = make_training_set(args.trajectories, args.steps, args.every, args.per_snapshot, log=log)
+        train_ks = tuple(int(k) for k in args.ks.split(","))
+        x, t = make_training_set(args.trajectories, args.steps, args.every, args.per_snapshot, ks=train_ks, log=log)
         log(f"training set: {len(x)} samples in {time.perf_counter() - t0:.1f} s")
         net, info = train(x, t, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
         Path(args.out).parent.mkdir(parents=True, exist_ok=True)
         net.save(args.out)
         info["trajectories"] = args.trajectories
         info["steps_per_trajectory"] = args.steps
+        info["ks"] = list(train_ks)
         info["wall_s"] = time.perf_counter() - t0
         log(json.dumps(info))
         if args.log:
