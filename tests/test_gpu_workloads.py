@@ -10,7 +10,7 @@ restatement:
 * the learned closure on the 1M block with the committed network: its plain limits bit for bit
   (a zero network equals plain steps of K dt, a judge that rejects every step equals plain steps
   of dt), its inputs against a numpy reference on sampled rows, and what the trained network
-  gains over a plain step of K dt."""
+  gains over a plain step of K dt, per step and over the whole run at K = 8."""
 import ctypes as C
 
 import numpy as np
@@ -292,3 +292,22 @@ def test_trained_closure_gains_over_plain_large_step():
              f"pos {res[smooth]['pos']:.3f} (interior particles {[x['interior'] for x in r['per_start']]})")
     assert res[False]["vel"] < 0.9
     assert res[True]["vel"] < 1.0
+
+
+@pytest.mark.timeout(900)
+def test_trained_closure_holds_the_peak_density_at_k8():
+    """What the committed network gains over the run: the 1M block over its whole 500-step horizon
+    in judged learned steps of 8 dt (each particle's own correction, judge 1 % per dt), against
+    plain steps of 8 dt and the plain dt run itself.  The plain large step overshoots the small-dt
+    run's peak density by more than 1 %; the learned step must stay within 1 % of it, while taking
+    8 dt per step (at least twice the plain dt run's simulated time per second)."""
+    from watercube_amd import closure
+
+    net = _trained_net()
+    res = closure.evaluate(net, "block_1m", horizon=500, ks=(8,), judge_tol=0.01, smooth=False, log=_log)["K8"]
+    plain, learned = res["plain_large"], res["learned"]
+    _log(f"K=8 peak-density excess: plain {plain['peak_excess']:+.4f}, learned {learned['peak_excess']:+.4f} "
+         f"({learned['fallbacks']} fallbacks, {learned['speedup_vs_plain']:.2f}x the plain dt run)")
+    assert plain["peak_excess"] > 0.01
+    assert abs(learned["peak_excess"]) <= 0.01
+    assert learned["speedup_vs_plain"] >= 2.0
