@@ -585,7 +585,9 @@ __global__ __launch_bounds__(BLOCK) void k_density(Scenes S, float4* __restrict_
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = scene_of(scene_table<BLOCK>(S), S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
-    const StepConsts& c = d.c;
+    // the constants by value: read through a reference, every LDS store of the neighbour loop
+    // could alias them (generic pointers may point into LDS), and they were reloaded per chunk
+    const StepConsts c = d.c;
     const uint32_t lt = (b - first_blocks<BLOCK>(S)[s]) * BLOCK + threadIdx.x;
     if (lt - __lane_id() >= c.n) return;  // the whole wave is past the scene's last particle
     const bool active = lt < c.n;
@@ -841,7 +843,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Scenes S, const float4* __rest
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = scene_of(scene_table<BLOCK>(S), S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
-    const StepConsts& c = d.c;
+    const StepConsts c = d.c;  // by value, as in k_density: not reloaded after each LDS store
     const uint32_t lt = (b - first_blocks<BLOCK>(S)[s]) * BLOCK + threadIdx.x;
     if (lt >= c.n) return;
     const uint32_t t = d.p_off + lt;
