@@ -245,8 +245,8 @@ def test_closure_1m_inputs_match_numpy_on_sampled_rows(oracle):
 @pytest.mark.timeout(600)
 def test_closure_1m_inference_from_lists_equals_features_then_network():
     """The learned step's inference at 1M after 300 steps, with the kinds of row of one state:
-    lists of at most 32 (all kept), of 33-40 and of 41-64 (the nearest 32 selected in registers,
-    two sizes of the selection).  Inputs formed inside the inference kernel must give the bits of
+    lists of at most 32 (all kept), of 33-40 (the nearest 32 selected by each lane in registers)
+    and of 41-64 (selected by the whole wave, one row at a time).  Inputs formed inside the inference kernel must give the bits of
     the feature matrix followed by the network, on every row.  (Overflowed lists, selected by the
     walk, are covered by the dense 16k scene of tests/test_gpu_closure.py.)"""
     p, parts, dt, _ = scenes.workload("block_1m")
@@ -272,7 +272,7 @@ def test_closure_1m_inference_from_lists_equals_features_then_network():
     for d in (x, y1, y2):
         d.free()
     net.close()
-    assert mid > 0 and long_ > 0  # both register selections (up to 40 and up to 64 entries) ran
+    assert mid > 0 and long_ > 0  # both selections (a lane's, up to 40 entries; the wave's) ran
     bad = np.nonzero((a.view(np.uint32) != b.view(np.uint32)).any(axis=1))[0]
     assert bad.size == 0, f"{bad.size} rows differ, first {bad[0]} (listed {listed[bad[0]]})"
 

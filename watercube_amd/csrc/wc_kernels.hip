@@ -1163,82 +1163,164 @@ __global__ void k_clear_word(uint32_t* __restrict__ w) {
     if (threadIdx.x == 0) *w = 0u;
 }
 
-// Lists a lane selects from in registers (k_closure_select) hold up to kSelRegCap or, rarely,
-// kNeighbourListMax entries; lists that overflowed go through k_closure_features on kHeavyBlocks
-// workgroups.
+// Lists a lane selects from in registers (k_closure_select) hold up to kSelRegCap entries; the
+// wave selects from longer ones together, and lists that overflowed go through k_closure_features
+// on kHeavyBlocks workgroups.
 constexpr int kSelRegCap = 40;
 constexpr int kHeavyBlocks = 64;
 static_assert(kSelRegCap <= (int)kNeighbourListMax, "the selection's loads stay inside the lists");
 
-// The kClosureNeighbours nearest of the m <= CAP listed neighbours of slot t, in list order, to
-// sel_out at the particle's slot: all list entries load at once, then the positions in groups of
-// 8 (the next group's gathers in flight while one is formed), and the m - kClosureNeighbours
-// farthest are dropped in registers -- the largest squared distance (by its bits, so NaN ranks
-// farthest), of equal ones the later entry.  That keeps the same neighbours as k_closure_features'
-// walk, which drops the same largest entries a few at a time (an entry dropped early is never
-// among the nearest of a larger set).
+// Loads at a wave-uniform base plus a per-lane 32-bit byte offset, the form global loads take with
+// the base in scalar registers (one VGPR per address instead of two).  Slots are below 2^28.
+__device__ __forceinline__ uint32_t load_at(const uint32_t* base, uint32_t slot) {
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + (slot << 2));
+}
+__device__ __forceinline__ float3 load_pos_at(const float4* base, uint32_t slot) {
+    const float* f = reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (slot << 4));
+    return float3{f[0], f[1], f[2]};
+}
+
+// The kClosureNeighbours nearest of the m listed neighbours of slot t, kClosureNeighbours < m <=
+// CAP <= kClosureNeighbours + 8, in list order, to sel_out at the particle's slot.
+// - The list entries load 8 at a time, two groups ahead of the neighbours' positions, which load 8
+//   at a time, one group ahead of the distances formed from them.  The key of an entry is its
+//   squared distance's bits, so NaN ranks farthest; entries past m take key 0.
+// - The D = m - kClosureNeighbours largest keys, each as often as it occurs, come out of an
+//   insertion network of 8 (two instructions per key and level, no indexing); the D-th largest, T,
+//   is the threshold.  Every key above T is dropped, and of the keys equal to T the last
+//   D - #{above} in list order: what dropping the largest key, of equal ones the later entry, D
+//   times drops.
+// - The entries are read again, the dropped ones removed in registers (each closes its gap, the
+//   last first), and the kClosureNeighbours kept ones written in list order, entry j of slot t at
+//   sel_out[j * stride + t]: every store of the wave is one contiguous run.
 template <int CAP>
-__device__ __forceinline__ void select_in_registers(uint32_t t, uint32_t m, const float4* __restrict__ posrho,
+__device__ __forceinline__ void select_by_threshold(uint32_t t, uint32_t m, uint32_t n,
+                                                    const float4* __restrict__ posrho,
                                                     const uint32_t* __restrict__ nlist, size_t stride,
                                                     uint32_t* __restrict__ sel_out) {
-    static_assert(CAP <= (int)kNeighbourListMax, "the loads stay inside the lists");
-    // every entry of the first CAP columns (entries past m are replaced by the particle's own slot
-    // after loading)
-    uint32_t u[CAP];
-#pragma unroll
-    for (int k = 0; k < CAP; ++k) u[k] = nlist[(size_t)k * stride + t];
-#pragma unroll
-    for (int k = 0; k < CAP; ++k) u[k] = (uint32_t)k < m ? u[k] : t;
+    constexpr int kTop = 8, kGroup = 8, kGroups = CAP / kGroup;
+    static_assert(CAP <= (int)kNeighbourListMax && CAP - kClosureNeighbours <= kTop, "one network of 8");
+    static_assert(CAP % kGroup == 0 && kGroups >= 2, "whole groups");
     const float4 p = posrho[t];
-    constexpr int kGroup = 8;
-    static_assert(CAP % kGroup == 0, "whole groups");
-    uint32_t key[CAP];
-    float3 cur[kGroup], nxt[kGroup];
+    // entries past m may be stale: one out of range is replaced by the particle's own slot
+    auto entries = [&](int g, uint32_t (&u)[kGroup]) {
 #pragma unroll
-    for (int j = 0; j < kGroup; ++j) {
-        const float* f = reinterpret_cast<const float*>(posrho + u[j]);
-        cur[j] = float3{f[0], f[1], f[2]};
-    }
+        for (int j = 0; j < kGroup; ++j) u[j] = load_at(nlist + (size_t)(g * kGroup + j) * stride, t);
+    };
+    auto gather = [&](int g, const uint32_t (&u)[kGroup], float3 (&q)[kGroup]) {
 #pragma unroll
-    for (int g = 0; g < CAP / kGroup; ++g) {
-        if (g + 1 < CAP / kGroup) {
-#pragma unroll
-            for (int j = 0; j < kGroup; ++j) {
-                const float* f = reinterpret_cast<const float*>(posrho + u[(g + 1) * kGroup + j]);
-                nxt[j] = float3{f[0], f[1], f[2]};
-            }
+        for (int j = 0; j < kGroup; ++j) {
+            q[j] = load_pos_at(posrho, u[j] < n ? u[j] : t);
         }
+    };
+    uint32_t key[CAP];
+    uint32_t ua[kGroup];
+    float3 cur[kGroup], nxt[kGroup];
+    entries(0, ua);
+    gather(0, ua, cur);
+    entries(1, ua);
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+        if (g + 1 < kGroups) gather(g + 1, ua, nxt);
+        if (g + 2 < kGroups) entries(g + 2, ua);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) {
             const float rx = p.x - cur[j].x, ry = p.y - cur[j].y, rz = p.z - cur[j].z;
-            key[g * kGroup + j] = __float_as_uint((rx * rx + ry * ry) + rz * rz);
+            uint32_t kk = __float_as_uint((rx * rx + ry * ry) + rz * rz);
+            kk = (uint32_t)(g * kGroup + j) < m ? kk : 0u;
+            // keeps the distance here, between the group's loads: left free, the instruction
+            // selector moves all of them after the last group, with every position held meanwhile
+            asm volatile("" : "+v"(kk));
+            key[g * kGroup + j] = kk;
         }
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) cur[j] = nxt[j];
     }
-    uint64_t gone = 0;
-    for (uint32_t d = (uint32_t)kClosureNeighbours; d < m; ++d) {
-        uint32_t best = 0;
-        int at = 0;
+    // top[0] >= top[1] >= ... >= top[7]: the 8 largest keys, each as often as it occurs
+    uint32_t top[kTop];
 #pragma unroll
-        for (int k = 0; k < CAP; ++k) {
-            const bool ge = (uint32_t)k < m && !((gone >> k) & 1ull) && key[k] >= best;
-            best = ge ? key[k] : best;
-            at = ge ? k : at;
+    for (int i = 0; i < kTop; ++i) top[i] = 0u;
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) {
+        uint32_t x = key[k];
+#pragma unroll
+        for (int i = 0; i < kTop; ++i) {
+            const uint32_t hi = max(top[i], x);
+            x = min(top[i], x);
+            top[i] = hi;
         }
-        gone |= 1ull << at;
     }
-    uint32_t* o = sel_out + (size_t)t * (kClosureFeatures / 2);
-    uint32_t w = 0;
+    const uint32_t drop = m - (uint32_t)kClosureNeighbours;  // 1 ... CAP - kClosureNeighbours
+    uint32_t thr = top[0];
 #pragma unroll
-    for (int k = 0; k < CAP; ++k)
-        if ((uint32_t)k < m && !((gone >> k) & 1ull)) o[w++] = u[k];
+    for (int i = 1; i < kTop; ++i) thr = drop == (uint32_t)(i + 1) ? top[i] : thr;
+    uint32_t above = 0;  // entries past m (key 0) are never above
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) above += key[k] > thr ? 1u : 0u;
+    // Keys equal to T are dropped from the end of the list.  With T = 0 the entries past m, last
+    // in the list and of key 0, come first: they take CAP - m of the drops, on top of the real ones.
+    uint32_t ties = drop - above + (thr == 0u ? (uint32_t)CAP - m : 0u);
+    uint64_t gone = 0;
+#pragma unroll
+    for (int k = CAP - 1; k >= 0; --k) {
+        const bool eq = key[k] == thr && ties != 0u;
+        ties -= eq ? 1u : 0u;
+        gone |= (key[k] > thr || eq) ? (1ull << k) : 0ull;
+    }
+    // the dropped entries before m (entries past m follow every kept one and need no removal)
+    uint64_t dropped = gone & ((1ull << m) - 1ull);  // m <= CAP < 64
+    __builtin_amdgcn_sched_barrier(0);
+    // read the entries again rather than hold them through the selection (registers); the pointer
+    // is passed through an empty asm statement so the loads are not merged with the first ones
+    const uint32_t* again = nlist;
+    asm volatile("" : "+s"(again));
+    uint32_t u[CAP];
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) u[k] = load_at(again + (size_t)k * stride, t);
+    // at most CAP - kClosureNeighbours drops; a lane with fewer shifts nothing in the later rounds
+#pragma unroll
+    for (int r = 0; r < CAP - kClosureNeighbours; ++r) {
+        if (__ballot(dropped != 0) == 0) break;  // wave-uniform
+        const uint32_t g = dropped != 0 ? 63u - (uint32_t)__clzll((long long)dropped) : 64u;
+        dropped &= ~(1ull << (g & 63u));
+#pragma unroll
+        for (int i = 0; i + 1 < CAP; ++i) u[i] = (uint32_t)i >= g ? u[i + 1] : u[i];
+    }
+#pragma unroll
+    for (int j = 0; j < kClosureNeighbours; ++j) sel_out[(size_t)j * stride + t] = u[j];
+}
+
+// The selection for one row of kSelRegCap + 1 ... kNeighbourListMax entries (m, uniform), made by
+// the whole wave: lane k takes entry k and ranks it among the row's m entries by (squared distance
+// bits, list position), reading the others' keys lane by lane.  The kClosureNeighbours lowest ranks
+// are kept, in list order -- the entries that dropping the largest key, of equal ones the later
+// entry, m - kClosureNeighbours times leaves.
+__device__ __forceinline__ void select_by_wave(uint32_t t, uint32_t m, const float4* __restrict__ posrho,
+                                               const uint32_t* __restrict__ nlist, size_t stride,
+                                               uint32_t* __restrict__ sel_out) {
+    static_assert(kNeighbourListMax == 64, "one entry per lane");
+    const uint32_t lane = threadIdx.x & 63;
+    const bool real = lane < m;
+    const uint32_t u = real ? nlist[(size_t)lane * stride + t] : t;
+    const float4 p = posrho[t];
+    const float* f = reinterpret_cast<const float*>(posrho + u);
+    const float rx = p.x - f[0], ry = p.y - f[1], rz = p.z - f[2];
+    const uint32_t key = __float_as_uint((rx * rx + ry * ry) + rz * rz);
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < m; ++j) {
+        const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)j);
+        rank += (kj < key || (kj == key && j < lane)) ? 1u : 0u;
+    }
+    const bool keep = real && rank < (uint32_t)kClosureNeighbours;
+    const uint64_t km = __ballot(keep);
+    if (keep) sel_out[(size_t)__popcll(km & ((1ull << lane) - 1ull)) * stride + t] = u;
 }
 
 // Selection of the kept neighbours of every slot whose list is longer than kClosureNeighbours:
-// in registers for lists of up to kNeighbourListMax, while overflowed lists are appended to `heavy`
-// (one atomic per wave that has any) for k_closure_features' walk.  Rows of at most
+// lists of up to kSelRegCap entries by each lane in registers, longer ones up to kNeighbourListMax
+// by the wave together, one row at a time, while overflowed lists are appended to `heavy` (one
+// atomic per wave that has any) for k_closure_features' walk.  Rows of at most
 // kClosureNeighbours need no selection.
 __global__ __launch_bounds__(256) void k_closure_select(const float4* __restrict__ posrho, uint32_t n,
                                                         const uint32_t* __restrict__ nlist,
@@ -1257,11 +1339,17 @@ __global__ __launch_bounds__(256) void k_closure_select(const float4* __restrict
         base = __shfl(base, (int)leader);
         if (is_heavy) heavy[base + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull))] = t;
     }
-    if (m <= (uint32_t)kClosureNeighbours || is_heavy) return;
-    if (m <= (uint32_t)kSelRegCap)
-        select_in_registers<kSelRegCap>(t, m, posrho, nlist, stride, sel_out);
-    else
-        select_in_registers<(int)kNeighbourListMax>(t, m, posrho, nlist, stride, sel_out);
+    if (m > (uint32_t)kClosureNeighbours && m <= (uint32_t)kSelRegCap)
+        select_by_threshold<kSelRegCap>(t, m, n, posrho, nlist, stride, sel_out);
+    // every lane of the wave takes part in each longer row
+    uint64_t rows = __ballot(m > (uint32_t)kSelRegCap && !is_heavy);
+    while (rows != 0) {  // wave-uniform
+        const int l = __ffsll((unsigned long long)rows) - 1;
+        rows &= rows - 1;
+        const uint32_t tr = (uint32_t)__builtin_amdgcn_readlane((int)t, l);
+        const uint32_t mr = (uint32_t)__builtin_amdgcn_readlane((int)m, l);
+        select_by_wave(tr, mr, posrho, nlist, stride, sel_out);
+    }
 }
 
 // Rows whose particle has at most kClosureNeighbours listed neighbours keep all of them, in list
@@ -1365,10 +1453,9 @@ __device__ __forceinline__ void closure_row(const Scenes& S, const float4* __res
     if (cnt > kClosureNeighbours) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
 
     if (sel_out) {
-        // the kept neighbours' slots instead of the row, at the particle's own slot, for inference
-        // that forms the values itself
-        uint32_t* o = sel_out + (size_t)(t - d.p_off) * (kClosureFeatures / 2);
-        for (int k = 0; k < cnt; ++k) o[k] = slot[k * kFeatBlock];
+        // the kept neighbours' slots instead of the row, entry k of slot t at sel_out[k * stride + t]
+        // as the selection writes them, for inference that forms the values itself
+        for (int k = 0; k < cnt; ++k) sel_out[(size_t)k * S.total_particles + (t - d.p_off)] = slot[k * kFeatBlock];
         return;
     }
     write_feature_row(row, c, t, p, pv, cnt, [&](int k) { return slot[k * kFeatBlock]; }, posrho, velp, dt_mult,
