@@ -205,6 +205,61 @@ def test_inference_from_lists_equals_features_then_network(scene):
     assert np.abs(a).max() > 0
 
 
+def test_lattice_ties_select_like_the_walk(oracle):
+    """An unjittered lattice at exactly representable coordinates (spacing 1/64, h = 2.3
+    spacings): squared distances come in shells of exactly equal values, so the 32 nearest of a
+    list of 34 or 38 (the per-lane selection) or of 41-56 (the wave's) are decided by the tie rule,
+    the earlier entry in enumeration order kept.  The fused inference's selection must give the
+    bits of the walk's features followed by the network, and the walk's rows must equal the numpy
+    reference's on every row that selects."""
+    p, _ = scenes.scaled_block((14, 13, 12), 1.0 / 64, seed=0, h_over_s=2.3)
+    parts = scenes.block(14, 13, 12, 1.0 / 64, 0.0, 0, origin=(0.25, 0.25, 0.25))
+    n = p.num_particles
+    net = ClosureNet(max_batch=64)
+    net.set_params((np.random.default_rng(13).normal(0, 0.05, ClosureNet.num_params())).astype(np.float32))
+    K = 4.0
+    with Fluid(p, parts) as f:
+        f.set_tuning(2, 2, 1)
+        f.sort_.run()
+        f.runDensityProg()
+        listed = f.neighbour_counts()
+        x = DeviceArray((n, 208), np.uint16)
+        y1 = DeviceArray((n, 4), np.float32)
+        y2 = DeviceArray((n, 4), np.float32)
+        L.check(L.lib().wc_closure_infer(f.ctx, net.h, C.c_float(K), y1.ptr), "wc_closure_infer")
+        L.check(L.lib().wc_closure_features(f.ctx, C.c_float(K), None, 0, x.ptr), "wc_closure_features")
+        L.check(L.lib().wc_mlp_predict(net.h, x.ptr, y2.ptr, n), "wc_mlp_predict")
+        a, b, got = y1.download(), y2.download(), x.download()
+        s = f.particles(2)
+        counts, offsets = f.sort_.getCountBuffer(), f.sort_.getOffsetBuffer()
+    kinds = {m: int((listed == m).sum()) for m in np.unique(listed)}
+    print(f"lattice: {n} particles, listed neighbours {kinds}", flush=True)
+    assert ((listed > 32) & (listed <= 40)).sum() > 0 and ((listed > 40) & (listed <= 64)).sum() > 0
+    bad = np.nonzero((a.view(np.uint32) != b.view(np.uint32)).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
+    f32 = np.float32
+    h = f32(p.kernel_radius)
+    inv_h = f32(1.0) / h
+    pos, rho = s["position"], s["density"]
+    for t in np.nonzero(listed > 32)[0]:
+        cand = []
+        for e, u in enumerate(_enumeration(oracle, p, pos, counts, offsets, t)):
+            d = pos[t] - pos[u]
+            d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+            if u != t and np.sqrt(d2) < h:
+                cand.append((d2, e, u))
+        assert len(cand) == listed[t]
+        cand.sort(key=lambda c: (c[0].view(np.uint32), c[1]))
+        cand = sorted(cand[:32], key=lambda c: c[1])
+        want = np.zeros(208, np.float32)
+        for k, (_, _, u) in enumerate(cand):
+            want[6 * k:6 * k + 3] = (pos[u] - pos[t]) * inv_h
+        want[192] = (rho[t] / f32(p.rest_density) - f32(1.0)) * f32(10.0)
+        want[193] = f32(K) * f32(0.25)
+        want[194] = f32(len(cand)) * f32(1.0 / 32)
+        assert np.array_equal(got[t], _bf16_bits(want)), f"slot {t} ({listed[t]} listed) differs"
+
+
 @pytest.mark.parametrize("scene", ["dam_break_16k", "ragged"])
 def test_closure_run_identical_with_and_without_lists(scene):
     """A learned run with a random network: with neighbour lists the macro step's closure rows come
