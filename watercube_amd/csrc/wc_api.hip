@@ -1467,6 +1467,8 @@ WC_API int wc_closure_features(wc_ctx* c, float dt_mult, const uint32_t* rows_ho
 
 // Inference on the closure inputs formed from the neighbour lists (no feature matrix): the
 // selection of longer lists into cl_x's memory, then the network.
+static_assert(wc::kSelRow == wc::kClosureNeighbours, "the selection writes rows the inference reads");
+static_assert(wc::kSelRow <= wc::kClosureFeatures / 2, "selection rows fit in the feature buffer");
 static hipError_t closure_infer_from_lists(wc_ctx* c, wc_mlp* net, float dt_mult, float* y) {
     const uint32_t n = c->total_particles;
     hipError_t e = wc::launch_closure_selection(c->b, n, reinterpret_cast<uint32_t*>(c->cl_x), c->cl_long,

@@ -1192,7 +1192,7 @@ __device__ __forceinline__ float3 load_pos_at(const float4* base, uint32_t slot)
 //   times drops.
 // - The entries are read again, the dropped ones removed in registers (each closes its gap, the
 //   last first), and the kClosureNeighbours kept ones written in list order, entry j of slot t at
-//   sel_out[j * stride + t]: every store of the wave is one contiguous run.
+//   sel_out[t * kClosureNeighbours + j]: one whole 128-byte line per row, in 16-byte stores.
 template <int CAP>
 __device__ __forceinline__ void select_by_threshold(uint32_t t, uint32_t m, uint32_t n,
                                                     const float4* __restrict__ posrho,
@@ -1287,8 +1287,9 @@ __device__ __forceinline__ void select_by_threshold(uint32_t t, uint32_t m, uint
 #pragma unroll
         for (int i = 0; i + 1 < CAP; ++i) u[i] = (uint32_t)i >= g ? u[i + 1] : u[i];
     }
+    uint4* row = reinterpret_cast<uint4*>(sel_out + (size_t)t * kClosureNeighbours);
 #pragma unroll
-    for (int j = 0; j < kClosureNeighbours; ++j) sel_out[(size_t)j * stride + t] = u[j];
+    for (int j = 0; j < kClosureNeighbours / 4; ++j) row[j] = make_uint4(u[4 * j], u[4 * j + 1], u[4 * j + 2], u[4 * j + 3]);
 }
 
 // The selection for one row of kSelRegCap + 1 ... kNeighbourListMax entries (m, uniform), made by
@@ -1314,7 +1315,7 @@ __device__ __forceinline__ void select_by_wave(uint32_t t, uint32_t m, const flo
     }
     const bool keep = real && rank < (uint32_t)kClosureNeighbours;
     const uint64_t km = __ballot(keep);
-    if (keep) sel_out[(size_t)__popcll(km & ((1ull << lane) - 1ull)) * stride + t] = u;
+    if (keep) sel_out[(size_t)t * kClosureNeighbours + (uint32_t)__popcll(km & ((1ull << lane) - 1ull))] = u;
 }
 
 // Selection of the kept neighbours of every slot whose list is longer than kClosureNeighbours:
@@ -1453,9 +1454,10 @@ __device__ __forceinline__ void closure_row(const Scenes& S, const float4* __res
     if (cnt > kClosureNeighbours) cnt = drop_farthest(key, slot, cnt, cnt - kClosureNeighbours);
 
     if (sel_out) {
-        // the kept neighbours' slots instead of the row, entry k of slot t at sel_out[k * stride + t]
-        // as the selection writes them, for inference that forms the values itself
-        for (int k = 0; k < cnt; ++k) sel_out[(size_t)k * S.total_particles + (t - d.p_off)] = slot[k * kFeatBlock];
+        // the kept neighbours' slots instead of the row, entry k of slot t at
+        // sel_out[t * kClosureNeighbours + k] as the selection writes them, for inference that
+        // forms the values itself
+        for (int k = 0; k < cnt; ++k) sel_out[(size_t)(t - d.p_off) * kClosureNeighbours + k] = slot[k * kFeatBlock];
         return;
     }
     write_feature_row(row, c, t, p, pv, cnt, [&](int k) { return slot[k * kFeatBlock]; }, posrho, velp, dt_mult,

@@ -528,10 +528,15 @@ __device__ __forceinline__ void fetch_entries(RowFetch& f, const ClosureRows& cr
     f.listed = f.listed_next;
     f.p = cr.posrho[f.row];
     f.pv = load3(cr.velp, f.row);
-    // the list and the selection share one layout: one load from a selected base, no branch
-    const uint32_t* src = (f.listed > 32u ? cr.sel : cr.nlist) + f.row;
+    // one load from a selected address rather than a branch per source
+    const bool sel = f.listed > 32u;
 #pragma unroll
-    for (int i = 0; i < kLaneNeighbours; ++i) f.e[i] = src[(size_t)(kLaneNeighbours * q4 + i) * cr.stride];
+    for (int i = 0; i < kLaneNeighbours; ++i) {
+        const uint32_t k = (uint32_t)(kLaneNeighbours * q4 + i);
+        const uint32_t* src = sel ? cr.sel + ((size_t)f.row * kSelRow + k)
+                                  : cr.nlist + ((size_t)k * cr.stride + f.row);
+        f.e[i] = *src;
+    }
 }
 
 // phase 2: the neighbours' records (entries past a list may be stale: a slot out of range is

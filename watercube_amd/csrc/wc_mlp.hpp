@@ -53,13 +53,14 @@ hipError_t mlp_infer(const Mlp& m, const uint16_t* X, float* Y, uint32_t n, hipS
 // values the closure's feature kernel would write: from a particle's neighbour list when it
 // holds at most 32, else from the 32 nearest its selection wrote to `sel`.  Single scene, row r =
 // slot r.
+constexpr int kSelRow = 32;  // slots per row of the selection buffer (the closure's neighbours)
 struct ClosureRows {
     const float4* posrho;    // sorted records: position, density
     const float4* velp;      // velocity, pressure
     const uint32_t* nlist;   // entry k of slot t at nlist[k * stride + t]
     const uint32_t* ncount;  // listed neighbours of slot t
-    const uint32_t* sel;     // more than 32 listed: the 32 nearest, in list order, at sel[k * stride + t]
-    size_t stride;           // of the lists and the selection
+    const uint32_t* sel;     // more than 32 listed: the 32 nearest, in list order, at sel[t * kSelRow + k]
+    size_t stride;           // of the lists
     float h, rho0, dt_mult, inv_v;
 };
 hipError_t mlp_infer_closure(const Mlp& m, const ClosureRows& rows, float* Y, uint32_t n, hipStream_t s);
