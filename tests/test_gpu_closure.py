@@ -170,16 +170,20 @@ def test_features_match_numpy_reference(scene, lists, oracle):
         assert np.array_equal(got[r], _bf16_bits(want)), f"row {r} (slot {t}) differs"
 
 
-@pytest.mark.parametrize("scene", ["dam_break_16k", "dense_h3", "ragged"])
+@pytest.mark.parametrize("scene", ["dam_break_16k", "dense_h3", "h23", "ragged"])
 def test_inference_from_lists_equals_features_then_network(scene):
     """wc_closure_infer forms each row's values inside the inference kernel (short lists read
     straight, longer ones through the selection's slots); it must give the bits of
-    wc_closure_features followed by wc_mlp_predict.  dense_h3's lists all overflow; ragged has
-    9,177 particles, so the last 128-row tile is partial."""
+    wc_closure_features followed by wc_mlp_predict.  dense_h3's lists all overflow; h23 (h = 2.3
+    spacings) lists about 50 neighbours, so most waves select by bisection; ragged has 9,177
+    particles, so the last 128-row tile is partial."""
     if scene == "dam_break_16k":
         p, parts, dt, _ = scenes.workload("dam_break_16k")
     elif scene == "dense_h3":
         p, parts = scenes.scaled_block((24, 24, 24), 0.01, seed=5, h_over_s=3.0)
+        dt = 2e-4
+    elif scene == "h23":
+        p, parts = scenes.scaled_block((24, 24, 24), 0.01, seed=9, h_over_s=2.3)
         dt = 2e-4
     else:
         p, parts = scenes.scaled_block((23, 21, 19), 0.01, seed=7, h_over_s=2.0)

@@ -1163,9 +1163,9 @@ __global__ void k_clear_word(uint32_t* __restrict__ w) {
     if (threadIdx.x == 0) *w = 0u;
 }
 
-// Lists a lane selects from in registers (k_closure_select) hold up to kSelRegCap entries; the
-// wave selects from longer ones together, and lists that overflowed go through k_closure_features
-// on kHeavyBlocks workgroups.
+// Lists a lane selects from by its threshold network (k_closure_select) hold up to kSelRegCap
+// entries; longer ones are selected by bisection or by the wave, and lists that overflowed go
+// through k_closure_features on kHeavyBlocks workgroups.
 constexpr int kSelRegCap = 40;
 constexpr int kHeavyBlocks = 64;
 static_assert(kSelRegCap <= (int)kNeighbourListMax, "the selection's loads stay inside the lists");
@@ -1202,11 +1202,13 @@ __device__ __forceinline__ void select_by_threshold(uint32_t t, uint32_t m, uint
     static_assert(CAP <= (int)kNeighbourListMax && CAP - kClosureNeighbours <= kTop, "one network of 8");
     static_assert(CAP % kGroup == 0 && kGroups >= 2, "whole groups");
     const float4 p = posrho[t];
-    // entries past m may be stale: one out of range is replaced by the particle's own slot
+    // the list's columns in order, through one advancing scalar pointer (the empty asm keeps the
+    // compiler from holding every column's address in scalar registers)
     auto entries = [&](int g, uint32_t (&u)[kGroup]) {
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) u[j] = load_at(nlist + (size_t)(g * kGroup + j) * stride, t);
     };
+    // entries past m may be stale: one out of range is replaced by the particle's own slot
     auto gather = [&](int g, const uint32_t (&u)[kGroup], float3 (&q)[kGroup]) {
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) {
@@ -1292,42 +1294,144 @@ __device__ __forceinline__ void select_by_threshold(uint32_t t, uint32_t m, uint
     for (int j = 0; j < kClosureNeighbours / 4; ++j) row[j] = make_uint4(u[4 * j], u[4 * j + 1], u[4 * j + 2], u[4 * j + 3]);
 }
 
-// The selection for one row of kSelRegCap + 1 ... kNeighbourListMax entries (m, uniform), made by
-// the whole wave: lane k takes entry k and ranks it among the row's m entries by (squared distance
-// bits, list position), reading the others' keys lane by lane.  The kClosureNeighbours lowest ranks
-// are kept, in list order -- the entries that dropping the largest key, of equal ones the later
-// entry, m - kClosureNeighbours times leaves.
-__device__ __forceinline__ void select_by_wave(uint32_t t, uint32_t m, const float4* __restrict__ posrho,
-                                               const uint32_t* __restrict__ nlist, size_t stride,
-                                               uint32_t* __restrict__ sel_out) {
+// The kClosureNeighbours nearest of the m listed neighbours of slot t, kSelRegCap < m <=
+// kNeighbourListMax, in list order, to sel_out at the particle's slot (as select_by_threshold).
+// - Keys are formed as there (entries past m take the largest key, 0xFFFFFFFF).
+// - v, the kClosureNeighbours-th smallest key with multiplicity (the least v with at least
+//   kClosureNeighbours keys <= v), is found by bisection over the 32-bit key range: 32 rounds of
+//   one compare and one add per key, no indexing.
+// - Kept: every key below v, and of the keys equal to v the first kClosureNeighbours - #{below} in
+//   list order -- the kClosureNeighbours smallest by (key, list position), which is what dropping
+//   the largest key, of equal ones the later entry, m - kClosureNeighbours times leaves.  Entries
+//   past m follow every real one, so they are never among them.
+// - The entries are read again and written to the lane's row of LDS at its next free word, which
+//   advances only past a kept entry (a dropped entry is overwritten by the next kept one, or lands
+//   in the row's spare last word); the row's kClosureNeighbours words then go out as 16-byte
+//   stores, one 128-byte line.
+constexpr int kSelStageWords = kClosureNeighbours + 1;  // per lane: odd, so lanes' rows fall in different banks
+__device__ __forceinline__ void select_by_bisection(uint32_t t, uint32_t m, uint32_t n, const float4* __restrict__ posrho,
+                                                    const uint32_t* __restrict__ nlist, size_t stride,
+                                                    uint32_t* __restrict__ sel_out, lds_ptr<uint32_t> stage) {
+    constexpr int CAP = (int)kNeighbourListMax, kGroup = 4, kGroups = CAP / kGroup;  // 64 keys held: smaller groups
+    static_assert(CAP <= 64 && CAP % kGroup == 0, "one 64-bit mask, whole groups");
+    const float4 p = posrho[t];
+    // the list's columns in order, through one advancing scalar pointer (the empty asm keeps the
+    // compiler from holding every column's address in scalar registers)
+    const uint32_t* col = nlist;
+    auto entries = [&](int, uint32_t (&u)[kGroup]) {
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) {
+            u[j] = load_at(col, t);
+            col += stride;
+            asm volatile("" : "+s"(col));
+        }
+    };
+    auto gather = [&](const uint32_t (&u)[kGroup], float3 (&q)[kGroup]) {
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) q[j] = load_pos_at(posrho, u[j] < n ? u[j] : t);
+    };
+    uint32_t key[CAP];
+    uint32_t ua[kGroup];
+    float3 cur[kGroup], nxt[kGroup];
+    entries(0, ua);
+    gather(ua, cur);
+    entries(1, ua);
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+        if (g + 1 < kGroups) gather(ua, nxt);
+        if (g + 2 < kGroups) entries(g + 2, ua);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) {
+            const float rx = p.x - cur[j].x, ry = p.y - cur[j].y, rz = p.z - cur[j].z;
+            uint32_t kk = __float_as_uint((rx * rx + ry * ry) + rz * rz);
+            kk = (uint32_t)(g * kGroup + j) < m ? kk : 0xFFFFFFFFu;
+            asm volatile("" : "+v"(kk));  // formed here, between the group's loads
+            key[g * kGroup + j] = kk;
+        }
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) cur[j] = nxt[j];
+    }
+    uint32_t lo = 0u, hi = 0xFFFFFFFFu;
+#pragma unroll 1
+    for (int round = 0; round < 32; ++round) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < CAP; ++k) c += key[k] <= mid ? 1u : 0u;
+        const bool enough = c >= (uint32_t)kClosureNeighbours;
+        hi = enough ? mid : hi;
+        lo = enough ? lo : mid + 1u;
+    }
+    const uint32_t v = lo;
+    uint32_t below = 0;
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) below += key[k] < v ? 1u : 0u;
+    uint32_t quota = (uint32_t)kClosureNeighbours - below;
+    uint64_t keep = 0;
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) {
+        const bool eq = key[k] == v && quota != 0u;
+        quota -= eq ? 1u : 0u;
+        keep |= (key[k] < v || eq) ? (1ull << k) : 0ull;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t* again = nlist;  // read again, not merged with the first loads (registers)
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) {
+        asm volatile("" : "+s"(again));
+        stage[w] = load_at(again, t);
+        again += stride;
+        w += (uint32_t)(keep >> k) & 1u;
+    }
+    uint4* row = reinterpret_cast<uint4*>(sel_out + (size_t)t * kClosureNeighbours);
+#pragma unroll
+    for (int j = 0; j < kClosureNeighbours / 4; ++j)
+        row[j] = make_uint4(stage[4 * j], stage[4 * j + 1], stage[4 * j + 2], stage[4 * j + 3]);
+}
+
+// One row of kSelRegCap + 1 ... kNeighbourListMax entries selected by the whole wave: slot t
+// (uniform), its m listed neighbours (uniform), the lane's entry u (entry `lane`; the row's own
+// slot past m) and that entry's position q.  Lane k ranks entry k among the row's m entries by
+// (squared distance bits, list position), reading the others' keys lane by lane; the
+// kClosureNeighbours lowest ranks are kept, in list order (the same entries as
+// select_by_bisection keeps).  For waves with a few such rows: one row costs the wave about as
+// much as the bisection costs each lane, but only the rows there are.
+constexpr int kSelWaveRowsMax = 8;
+struct WaveRow {
+    uint32_t t, m, u;
+    float3 q;
+};
+__device__ __forceinline__ void select_row_by_wave(const WaveRow& w, const float4* __restrict__ posrho,
+                                                   uint32_t* __restrict__ sel_out) {
     static_assert(kNeighbourListMax == 64, "one entry per lane");
     const uint32_t lane = threadIdx.x & 63;
-    const bool real = lane < m;
-    const uint32_t u = real ? nlist[(size_t)lane * stride + t] : t;
-    const float4 p = posrho[t];
-    const float* f = reinterpret_cast<const float*>(posrho + u);
-    const float rx = p.x - f[0], ry = p.y - f[1], rz = p.z - f[2];
+    const float4 p = posrho[w.t];
+    const float rx = p.x - w.q.x, ry = p.y - w.q.y, rz = p.z - w.q.z;
     const uint32_t key = __float_as_uint((rx * rx + ry * ry) + rz * rz);
     uint32_t rank = 0;
-    for (uint32_t j = 0; j < m; ++j) {
+    for (uint32_t j = 0; j < w.m; ++j) {
         const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)j);
         rank += (kj < key || (kj == key && j < lane)) ? 1u : 0u;
     }
-    const bool keep = real && rank < (uint32_t)kClosureNeighbours;
+    const bool keep = lane < w.m && rank < (uint32_t)kClosureNeighbours;
     const uint64_t km = __ballot(keep);
-    if (keep) sel_out[(size_t)t * kClosureNeighbours + (uint32_t)__popcll(km & ((1ull << lane) - 1ull))] = u;
+    if (keep) sel_out[(size_t)w.t * kClosureNeighbours + (uint32_t)__popcll(km & ((1ull << lane) - 1ull))] = w.u;
 }
 
 // Selection of the kept neighbours of every slot whose list is longer than kClosureNeighbours:
-// lists of up to kSelRegCap entries by each lane in registers, longer ones up to kNeighbourListMax
-// by the wave together, one row at a time, while overflowed lists are appended to `heavy` (one
-// atomic per wave that has any) for k_closure_features' walk.  Rows of at most
-// kClosureNeighbours need no selection.
+// lists of up to kSelRegCap entries by each lane's threshold network; longer ones up to
+// kNeighbourListMax by each lane's bisection in a wave that has more than kSelWaveRowsMax of them,
+// else by the whole wave one row at a time; overflowed lists are appended to `heavy` (one atomic
+// per wave that has any) for k_closure_features' walk.  Rows of at most kClosureNeighbours need no
+// selection.
 __global__ __launch_bounds__(256) void k_closure_select(const float4* __restrict__ posrho, uint32_t n,
                                                         const uint32_t* __restrict__ nlist,
                                                         const uint32_t* __restrict__ ncount, size_t stride,
                                                         uint32_t* __restrict__ sel_out, uint32_t* __restrict__ heavy,
                                                         uint32_t* __restrict__ heavy_count) {
+    __shared__ uint32_t s_stage[256 * kSelStageWords];
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     const uint32_t m = t < n ? ncount[t] : 0u;
     const bool is_heavy = m > kNeighbourListMax;  // kListOverflow
@@ -1342,14 +1446,47 @@ __global__ __launch_bounds__(256) void k_closure_select(const float4* __restrict
     }
     if (m > (uint32_t)kClosureNeighbours && m <= (uint32_t)kSelRegCap)
         select_by_threshold<kSelRegCap>(t, m, n, posrho, nlist, stride, sel_out);
-    // every lane of the wave takes part in each longer row
-    uint64_t rows = __ballot(m > (uint32_t)kSelRegCap && !is_heavy);
-    while (rows != 0) {  // wave-uniform
+    const bool longer = m > (uint32_t)kSelRegCap && !is_heavy;
+    uint64_t rows = __ballot(longer);
+    if (__popcll(rows) > kSelWaveRowsMax) {  // wave-uniform
+        if (longer)
+            select_by_bisection(t, m, n, posrho, nlist, stride, sel_out,
+                                (lds_ptr<uint32_t>)s_stage + threadIdx.x * kSelStageWords);
+        return;
+    }
+    // A few longer rows: every lane of the wave takes part in each, one after another, while the
+    // next row's positions and the one after's list entries are in flight.
+    const uint32_t lane = threadIdx.x & 63;
+    auto next_row = [&](WaveRow& w) {  // wave-uniform; issues the row's list loads
+        if (rows == 0) return false;
         const int l = __ffsll((unsigned long long)rows) - 1;
         rows &= rows - 1;
-        const uint32_t tr = (uint32_t)__builtin_amdgcn_readlane((int)t, l);
-        const uint32_t mr = (uint32_t)__builtin_amdgcn_readlane((int)m, l);
-        select_by_wave(tr, mr, posrho, nlist, stride, sel_out);
+        w.t = (uint32_t)__builtin_amdgcn_readlane((int)t, l);
+        w.m = (uint32_t)__builtin_amdgcn_readlane((int)m, l);
+        w.u = lane < w.m ? nlist[(size_t)lane * stride + w.t] : w.t;
+        return true;
+    };
+    auto positions = [&](WaveRow& w) {
+        const float* f = reinterpret_cast<const float*>(posrho + w.u);
+        w.q = float3{f[0], f[1], f[2]};
+    };
+    WaveRow a, b, c;
+    bool have_a = next_row(a), have_b = false;
+    if (have_a) {
+        positions(a);
+        have_b = next_row(b);
+    }
+    while (have_a) {  // wave-uniform
+        bool have_c = false;
+        if (have_b) {
+            positions(b);
+            have_c = next_row(c);
+        }
+        select_row_by_wave(a, posrho, sel_out);
+        a = b;
+        b = c;
+        have_a = have_b;
+        have_b = have_c;
     }
 }
 
