@@ -1459,7 +1459,7 @@ WC_API int wc_closure_features(wc_ctx* c, float dt_mult, const uint32_t* rows_ho
     } else {
         nrows = n;
     }
-    WC_HIP(wc::launch_closure_features(c->b, rows, nrows, dt_mult, 1.0f, (uint16_t*)x_dev, c->neighbour_list,
+    WC_HIP(wc::launch_closure_features(c->b, rows, nrows, dt_mult, (uint16_t*)x_dev, c->neighbour_list,
                                        c->stream));
     WC_HIP(hipStreamSynchronize(c->stream));
     return WC_OK;
@@ -1477,7 +1477,7 @@ static hipError_t closure_infer_from_lists(wc_ctx* c, wc_mlp* net, float dt_mult
     const wc::StepConsts& k = c->desc[0].c;
     const wc::ClosureRows rows{c->b.posrho[1], c->b.velp[1], c->b.nlist, c->b.ncount,
                                reinterpret_cast<const uint32_t*>(c->cl_x), (size_t)c->total_particles, k.h, k.rho0,
-                               dt_mult, 1.0f};
+                               dt_mult};
     return wc::mlp_infer_closure(net->m, rows, y, n, c->stream);
 }
 
@@ -1563,7 +1563,7 @@ WC_API int wc_closure_run(wc_ctx* c, wc_mlp* net, float dt, uint32_t K, uint32_t
         if (net && c->neighbour_list) {
             WC_HIP(closure_infer_from_lists(c, net, (float)K, c->cl_y));
         } else if (net) {
-            WC_HIP(wc::launch_closure_features(c->b, nullptr, n, (float)K, 1.0f, c->cl_x, c->neighbour_list,
+            WC_HIP(wc::launch_closure_features(c->b, nullptr, n, (float)K, c->cl_x, c->neighbour_list,
                                                c->stream));
             WC_HIP(wc::mlp_infer(net->m, c->cl_x, c->cl_y, n, c->stream));
         }

@@ -123,7 +123,7 @@ constexpr int kClosureFeatures = 208;  // 195 used, zero padded
 // Features of the sorted particles at `rows` (slots; NULL = every slot) into X (bf16 rows); with
 // `lists`, from the neighbour lists of the last density pass instead of a search.
 hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows, uint32_t nrows, float dt_mult,
-                                   float v_ref, uint16_t* X, bool lists, hipStream_t stream);
+                                   uint16_t* X, bool lists, hipStream_t stream);
 // Force pass with dt given directly; corr (4 floats per slot, units of g, may be NULL) is added to
 // the acceleration of particles at least 2h from every wall, averaged over each particle's
 // neighbours when `smooth`; acc_out (may be NULL) receives the plain acceleration.  substeps = K

@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(kHashBlock) void k_advect(Scenes S, float4* __restr
 // Learned closure features (single-scene contexts).  For every requested slot of the sorted
 // buffer: its 32 nearest neighbours inside the kernel radius (the density pass's acceptance
 // test; nearest by squared distance, among equal distances the earlier in NEIGHBORHOOD
-// enumeration order), listed in enumeration order, each as ((x_j - x_i) / h, (v_j - v_i) / v_ref);
+// enumeration order), listed in enumeration order, each as ((x_j - x_i) / h, v_j - v_i);
 // then rho_i / rho0 - 1 scaled by 10, the step multiplier / 4 and the fraction of the 32 places
 // filled; zero padded to 208 bf16 values.
 // ------------------------------------------------------------------------------------------
@@ -1099,7 +1099,7 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
-// One feature row: the kept neighbours' relative positions (/ h) and velocities (* inv_v), in
+// One feature row: the kept neighbours' relative positions (/ h) and velocities, in
 // bf16, then the scaled density deviation, dt multiplier and filled fraction.  slot_of(k) gives
 // the k-th kept neighbour's slot, k < cnt.
 // Software-pipelined: the gathers of group g + 1 are issued before group g is formed, and the
@@ -1109,7 +1109,7 @@ template <typename SlotOf>
 __device__ __forceinline__ void write_feature_row(uint4* __restrict__ row, const StepConsts& c, uint32_t t, float4 p,
                                                   float4 pv, int cnt, SlotOf slot_of,
                                                   const float4* __restrict__ posrho, const float4* __restrict__ velp,
-                                                  float dt_mult, float inv_v) {
+                                                  float dt_mult) {
     const float inv_h = 1.0f / c.h;
     float4 q[4], qv[4];
     auto load_group = [&](int g) {
@@ -1144,9 +1144,9 @@ __device__ __forceinline__ void write_feature_row(uint4* __restrict__ row, const
             v[6 * j + 0] = keep((cq[j].x - p.x) * inv_h);
             v[6 * j + 1] = keep((cq[j].y - p.y) * inv_h);
             v[6 * j + 2] = keep((cq[j].z - p.z) * inv_h);
-            v[6 * j + 3] = keep((cqv[j].x - pv.x) * inv_v);
-            v[6 * j + 4] = keep((cqv[j].y - pv.y) * inv_v);
-            v[6 * j + 5] = keep((cqv[j].z - pv.z) * inv_v);
+            v[6 * j + 3] = keep(cqv[j].x - pv.x);
+            v[6 * j + 4] = keep(cqv[j].y - pv.y);
+            v[6 * j + 5] = keep(cqv[j].z - pv.z);
         }
 #pragma unroll
         for (int e = 0; e < 3; ++e)
@@ -1496,7 +1496,7 @@ __global__ __launch_bounds__(256) void k_closure_select(const float4* __restrict
 __global__ __launch_bounds__(256) void k_closure_features_short(Scenes S, const float4* __restrict__ posrho,
                                                                 const float4* __restrict__ velp,
                                                                 const uint32_t* __restrict__ rows, uint32_t nrows,
-                                                                float dt_mult, float inv_v, uint16_t* __restrict__ X,
+                                                                float dt_mult, uint16_t* __restrict__ X,
                                                                 const uint32_t* __restrict__ nlist,
                                                                 const uint32_t* __restrict__ ncount) {
     const uint32_t r = blockIdx.x * 256 + threadIdx.x;
@@ -1516,15 +1516,14 @@ __global__ __launch_bounds__(256) void k_closure_features_short(Scenes S, const 
     uint32_t u[kClosureNeighbours];
 #pragma unroll
     for (int k = 0; k < kClosureNeighbours; ++k) u[k] = (uint32_t)k < listed ? nlist[(size_t)k * stride + t] : t;
-    write_feature_row(row, c, t, posrho[t], velp[t], (int)listed, [&](int k) { return u[k]; }, posrho, velp, dt_mult,
-                      inv_v);
+    write_feature_row(row, c, t, posrho[t], velp[t], (int)listed, [&](int k) { return u[k]; }, posrho, velp, dt_mult);
 }
 
 // One row of k_closure_features.
 __device__ __forceinline__ void closure_row(const Scenes& S, const float4* __restrict__ posrho,
                                             const float4* __restrict__ velp, const uint32_t* __restrict__ skey,
                                             const uint2* __restrict__ range, const uint32_t* __restrict__ rows,
-                                            uint32_t r, float dt_mult, float inv_v, uint16_t* __restrict__ X,
+                                            uint32_t r, float dt_mult, uint16_t* __restrict__ X,
                                             const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
                                             bool skip_short, uint32_t* __restrict__ sel_out, lds_ptr<uint32_t> key,
                                             lds_ptr<uint32_t> slot) {
@@ -1597,8 +1596,7 @@ __device__ __forceinline__ void closure_row(const Scenes& S, const float4* __res
         for (int k = 0; k < cnt; ++k) sel_out[(size_t)(t - d.p_off) * kClosureNeighbours + k] = slot[k * kFeatBlock];
         return;
     }
-    write_feature_row(row, c, t, p, pv, cnt, [&](int k) { return slot[k * kFeatBlock]; }, posrho, velp, dt_mult,
-                      inv_v);
+    write_feature_row(row, c, t, p, pv, cnt, [&](int k) { return slot[k * kFeatBlock]; }, posrho, velp, dt_mult);
 }
 
 __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const float4* __restrict__ posrho,
@@ -1606,7 +1604,7 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
                                                                  const uint32_t* __restrict__ skey,
                                                                  const uint2* __restrict__ range,
                                                                  const uint32_t* __restrict__ rows, uint32_t nrows,
-                                                                 float dt_mult, float inv_v, uint16_t* __restrict__ X,
+                                                                 float dt_mult, uint16_t* __restrict__ X,
                                                                  const uint32_t* __restrict__ nlist,
                                                                  const uint32_t* __restrict__ ncount,
                                                                  bool skip_short, uint32_t* __restrict__ sel_out,
@@ -1618,7 +1616,7 @@ __global__ __launch_bounds__(kFeatBlock) void k_closure_features(Scenes S, const
     if (nrows_dev) nrows = *nrows_dev;  // a count an earlier kernel wrote
     // rows r, r + the grid, ...: one row per lane when the grid covers them all
     for (uint32_t r = blockIdx.x * kFeatBlock + threadIdx.x; r < nrows; r += gridDim.x * kFeatBlock)
-        closure_row(S, posrho, velp, skey, range, rows, r, dt_mult, inv_v, X, nlist, ncount, skip_short, sel_out,
+        closure_row(S, posrho, velp, skey, range, rows, r, dt_mult, X, nlist, ncount, skip_short, sel_out,
                     (lds_ptr<uint32_t>)s_key + threadIdx.x, (lds_ptr<uint32_t>)s_slot + threadIdx.x);
 }
 
@@ -1802,17 +1800,17 @@ hipError_t launch_update(const DeviceBuffers& b, int v, bool list, hipStream_t s
 }
 
 hipError_t launch_closure_features(const DeviceBuffers& b, const uint32_t* rows, uint32_t nrows, float dt_mult,
-                                   float v_ref, uint16_t* X, bool lists, hipStream_t s) {
+                                   uint16_t* X, bool lists, hipStream_t s) {
     if (nrows == 0) return hipSuccess;
     if (lists && (!b.nlist || !b.ncount)) return hipErrorInvalidValue;
     if (lists) {
         hipLaunchKernelGGL(k_closure_features_short, dim3(blocks_for(nrows, 256)), dim3(256), 0, s, b.sc, b.posrho[1],
-                           b.velp[1], rows, nrows, dt_mult, 1.0f / v_ref, X, b.nlist, b.ncount);
+                           b.velp[1], rows, nrows, dt_mult, X, b.nlist, b.ncount);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_closure_features, dim3(blocks_for(nrows, kFeatBlock)), dim3(kFeatBlock), 0, s, b.sc,
-                       b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, 1.0f / v_ref, X,
+                       b.posrho[1], b.velp[1], b.skey, b.range, rows, nrows, dt_mult, X,
                        lists ? b.nlist : nullptr, lists ? b.ncount : nullptr, lists, nullptr, nullptr);
     return hipGetLastError();
 }
@@ -1831,7 +1829,7 @@ hipError_t launch_closure_selection(const DeviceBuffers& b, uint32_t n, uint32_t
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_closure_features, dim3(kHeavyBlocks), dim3(kFeatBlock), 0, s, b.sc, b.posrho[1], b.velp[1],
-                       b.skey, b.range, heavy, n, 0.0f, 1.0f, nullptr, b.nlist, b.ncount, true, sel, heavy_count);
+                       b.skey, b.range, heavy, n, 0.0f, nullptr, b.nlist, b.ncount, true, sel, heavy_count);
     return hipGetLastError();
 }
 

@@ -562,26 +562,32 @@ __device__ __forceinline__ void form_rows(uint4 (&a)[kKSteps1], const RowFetch& 
     const uint32_t cnt = f.listed > 32u ? 32u : f.listed;  // longer lists keep their 32 nearest
     const float inv_h = 1.0f / cr.h;
     float v[kLaneNeighbours][6];
+    uint32_t mask[kLaneNeighbours];
 #pragma unroll
     for (int i = 0; i < kLaneNeighbours; ++i) {
         // zeros past the list by masking the bits (an opaque mask keeps the loads unconditional)
-        uint32_t m = (uint32_t)(kLaneNeighbours * q4 + i) < cnt ? ~0u : 0u;
-        asm("" : "+v"(m));
-        auto keep = [m](float x) { return __uint_as_float(__float_as_uint(x) & m); };
-        v[i][0] = keep((f.q[i].x - f.p.x) * inv_h);
-        v[i][1] = keep((f.q[i].y - f.p.y) * inv_h);
-        v[i][2] = keep((f.q[i].z - f.p.z) * inv_h);
-        v[i][3] = keep((f.qv[i].x - f.pv.x) * cr.inv_v);
-        v[i][4] = keep((f.qv[i].y - f.pv.y) * cr.inv_v);
-        v[i][5] = keep((f.qv[i].z - f.pv.z) * cr.inv_v);
+        mask[i] = (uint32_t)(kLaneNeighbours * q4 + i) < cnt ? ~0u : 0u;
+        asm("" : "+v"(mask[i]));
+        v[i][0] = (f.q[i].x - f.p.x) * inv_h;
+        v[i][1] = (f.q[i].y - f.p.y) * inv_h;
+        v[i][2] = (f.q[i].z - f.p.z) * inv_h;
+        v[i][3] = f.qv[i].x - f.pv.x;
+        v[i][4] = f.qv[i].y - f.pv.y;
+        v[i][5] = f.qv[i].z - f.pv.z;
     }
+    // each packed pair belongs to one neighbour (6 values each, pairs start at even positions), so
+    // its mask applies to the pair: bf16 pairs of masked values, with half the masking
 #pragma unroll
     for (int s = 0; s < kFeatSteps; ++s) {
         const int first = (8 * s) / 6, o = (8 * s) % 6;  // compile-time
-        float c[8];
+        static_assert((8 * kFeatSteps) % 2 == 0 && 6 % 2 == 0, "pairs within a neighbour");
+        uint32_t w[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) c[e] = o + e < 6 ? v[first][o + e] : v[first + 1][o + e - 6];
-        a[s] = make_uint4(pack2bf(c[0], c[1]), pack2bf(c[2], c[3]), pack2bf(c[4], c[5]), pack2bf(c[6], c[7]));
+        for (int e = 0; e < 8; e += 2) {
+            const int i = o + e < 6 ? first : first + 1, k = o + e < 6 ? o + e : o + e - 6;
+            w[e / 2] = pack2bf(v[i][k], v[i][k + 1]) & mask[i];
+        }
+        a[s] = make_uint4(w[0], w[1], w[2], w[3]);
     }
     // columns 192..207: the density deviation, the step multiplier and the filled fraction
     const float rho_rel = (f.p.w / cr.rho0 - 1.0f) * 10.0f;

@@ -61,7 +61,7 @@ struct ClosureRows {
     const uint32_t* ncount;  // listed neighbours of slot t
     const uint32_t* sel;     // more than 32 listed: the 32 nearest, in list order, at sel[t * kSelRow + k]
     size_t stride;           // of the lists
-    float h, rho0, dt_mult, inv_v;
+    float h, rho0, dt_mult;
 };
 hipError_t mlp_infer_closure(const Mlp& m, const ClosureRows& rows, float* Y, uint32_t n, hipStream_t s);
 hipError_t mlp_forward_train(Mlp& m, const uint16_t* X, const float* T, float* Y, uint32_t M, hipStream_t s);
