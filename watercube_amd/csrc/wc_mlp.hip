@@ -528,25 +528,29 @@ __device__ __forceinline__ void fetch_entries(RowFetch& f, const ClosureRows& cr
     f.listed = f.listed_next;
     f.p = cr.posrho[f.row];
     f.pv = load3(cr.velp, f.row);
-    // one load from a selected address rather than a branch per source
+    // entry k of the row at base + k * step: the selection's row (step 1) or the list's column
+    // (step stride), one selected address advanced per entry rather than a branch per source
     const bool sel = f.listed > 32u;
+    const size_t step = sel ? (size_t)1 : cr.stride;
+    const uint32_t* src = (sel ? cr.sel + (size_t)f.row * kSelRow : cr.nlist + f.row) + (size_t)(kLaneNeighbours * q4) * step;
 #pragma unroll
     for (int i = 0; i < kLaneNeighbours; ++i) {
-        const uint32_t k = (uint32_t)(kLaneNeighbours * q4 + i);
-        const uint32_t* src = sel ? cr.sel + ((size_t)f.row * kSelRow + k)
-                                  : cr.nlist + ((size_t)k * cr.stride + f.row);
         f.e[i] = *src;
+        src += step;
     }
 }
 
 // phase 2: the neighbours' records (entries past a list may be stale: a slot out of range is
 // replaced by the row's own, so every load stays inside the buffers)
 __device__ __forceinline__ void fetch_records(RowFetch& f, const ClosureRows& cr, uint32_t n, int lane) {
+    // a scalar base plus one 32-bit byte offset for both records (slots are below 2^28)
 #pragma unroll
     for (int i = 0; i < kLaneNeighbours; ++i) {
-        const uint32_t u = f.e[i] < n ? f.e[i] : f.row;
-        f.q[i] = load3(cr.posrho, u);
-        f.qv[i] = load3(cr.velp, u);
+        const uint32_t off = (f.e[i] < n ? f.e[i] : f.row) << 4;
+        const float* a = reinterpret_cast<const float*>(reinterpret_cast<const char*>(cr.posrho) + off);
+        const float* b = reinterpret_cast<const float*>(reinterpret_cast<const char*>(cr.velp) + off);
+        f.q[i] = float3{a[0], a[1], a[2]};
+        f.qv[i] = float3{b[0], b[1], b[2]};
     }
 }
 
