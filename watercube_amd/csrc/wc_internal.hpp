@@ -31,9 +31,9 @@ constexpr uint32_t kNeighbourListMax = 64;
 constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
 // The reorder ranks a particle inside its cell by counting the cell's particles with a smaller
 // input index: at most kGatherDirectMax of them.  More crowded cells (a dense cluster, or every
-// NaN position, which hashes to cell 0) go to a work list, and one workgroup per listed cell sorts
-// its input indices instead (k_gather_crowded).  At most capacity / (kGatherDirectMax + 1) cells
-// can be listed.
+// NaN position, which hashes to cell 0) go to a work list, and one of the gather's leading
+// workgroups per listed cell sorts its input indices instead.  At most
+// capacity / (kGatherDirectMax + 1) cells can be listed.
 constexpr uint32_t kGatherDirectMax = 256;
 inline uint32_t crowd_capacity(uint64_t particles) { return (uint32_t)(particles / (kGatherDirectMax + 1) + 1); }
 

@@ -297,18 +297,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(Scenes S, uint32_t* __res
 // ------------------------------------------------------------------------------------------
 // 3. place: slot = first slot of the cell + arrival rank (reorder.comp:38-43 without the copy).
 // ------------------------------------------------------------------------------------------
+// The first particle to arrive in a cell of more than kGatherDirectMax also lists the cell for the
+// leading workgroups of k_gather, which sort its input indices instead of ranking them one by one.
 __global__ __launch_bounds__(kHashBlock) void k_place(Scenes S, const uint32_t* __restrict__ key,
                                                       const uint32_t* __restrict__ rank,
                                                       const uint2* __restrict__ range,
-                                                      uint32_t* __restrict__ slot_tmp) {
+                                                      uint32_t* __restrict__ slot_tmp, uint2* __restrict__ crowd,
+                                                      uint32_t* __restrict__ crowd_count, uint32_t crowd_cap,
+                                                      uint32_t* __restrict__ error_flag) {
     const uint32_t b = blockIdx.x;
     const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
     const SceneDesc& d = S.desc[s];
     const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
     if (lt >= d.c.n) return;
     const uint32_t i = d.p_off + lt;
-    const uint32_t slot = range[d.c_off + key[i]].x + rank[i];
+    const uint32_t k = key[i];
+    const uint2 rg = range[d.c_off + k];
+    const uint32_t r = rank[i];
+    const uint32_t slot = rg.x + r;
     if (slot - d.p_off < d.c.n) slot_tmp[slot] = i;  // always true when the scan succeeded
+    if (rg.y > kGatherDirectMax && r == 0u) {       // one entry per crowded cell
+        const uint32_t at = atomicAdd(crowd_count, 1u);
+        if (at < crowd_cap)
+            crowd[at] = make_uint2(d.c_off + k, s);
+        else
+            atomicOr(error_flag, 2u);  // cannot happen when the scan succeeded
+    }
 }
 
 // Correctly rounded a / b from y = RN(1 / b): q0 = RN(a y), r = a - b q0 (exact), RN(q0 + r y)
@@ -335,8 +349,8 @@ __device__ __forceinline__ bool div_operand(float x) {
 //    (number of particles of its cell with a smaller input index), so every cell keeps input
 //    order; the record is copied there (reorder.comp:44) and the cell index kept beside it.
 //    Counting costs one pass over the cell per particle, so only cells of at most
-//    kGatherDirectMax particles are ranked this way; the first slot of a more crowded cell lists
-//    it for k_gather_crowded, which sorts the cell's input indices instead.
+//    kGatherDirectMax particles are ranked this way.  k_place lists each more crowded cell, and
+//    the first kCrowdBlocks workgroups of k_gather sort its input indices instead.
 // ------------------------------------------------------------------------------------------
 // Moves the particle with input index i to slot dst of the sorted buffer (reorder.comp:44).
 __device__ __forceinline__ void move_record(uint32_t i, uint32_t dst, uint32_t k, const float4* __restrict__ posrho_in,
@@ -354,48 +368,8 @@ __device__ __forceinline__ void move_record(uint32_t i, uint32_t dst, uint32_t k
     if (!(div_operand(v.x) && div_operand(v.y) && div_operand(v.z))) atomicOr(div_flags, kDivUnsafe);
 }
 
-__global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, const uint32_t* __restrict__ slot_tmp,
-                                                       const uint32_t* __restrict__ key,
-                                                       const uint2* __restrict__ range,
-                                                       const float4* __restrict__ posrho_in,
-                                                       const float4* __restrict__ velp_in,
-                                                       float4* __restrict__ posrho_out, float4* __restrict__ velp_out,
-                                                       uint32_t* __restrict__ sorted, uint32_t* __restrict__ skey,
-                                                       const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id_out,
-                                                       uint32_t* __restrict__ div_flags, uint2* __restrict__ crowd,
-                                                       uint32_t* __restrict__ crowd_count, uint32_t crowd_cap,
-                                                       uint32_t* __restrict__ error_flag) {
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
-    const SceneDesc& d = S.desc[s];
-    const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
-    if (lt >= d.c.n) return;
-    const uint32_t t = d.p_off + lt;
-    const uint32_t lim = d.p_off + d.c.n;
-    const uint32_t i = slot_tmp[t];
-    if (i - d.p_off >= d.c.n) return;  // only after a failed scan (error flag set)
-    const uint32_t k = key[i];
-    const uint2 rg = range[d.c_off + k];
-    if (rg.y > kGatherDirectMax) {
-        if (t == rg.x) {  // one entry per crowded cell
-            const uint32_t at = atomicAdd(crowd_count, 1u);
-            if (at < crowd_cap)
-                crowd[at] = make_uint2(d.c_off + k, s);
-            else
-                atomicOr(error_flag, 2u);  // cannot happen when the scan succeeded
-        }
-        return;
-    }
-    const uint32_t end = min(rg.x + rg.y, lim);
-    uint32_t r = 0;
-    for (uint32_t u = rg.x; u < end; ++u) r += (slot_tmp[u] < i) ? 1u : 0u;
-    const uint32_t dst = rg.x + r;
-    if (dst >= lim) return;
-    move_record(i, dst, k, posrho_in, velp_in, posrho_out, velp_out, sorted, skey, id_in, id_out, div_flags);
-}
-
-// Crowded cells: one workgroup per listed cell (a grid-stride loop over the list, whose length the
-// gather pass left in crowd_count) sorts the cell's input indices, slot_tmp[first .. first + c),
+// Crowded cells: one workgroup per listed cell (a loop over the list with stride nblocks, whose
+// length k_place left in crowd_count) sorts the cell's input indices, slot_tmp[first .. first + c),
 // in ascending order: chunks of kCrowdChunk by a bitonic sort in LDS, then merge passes between
 // slot_tmp and sorted over the same slots (each element's place in the merged run is its index in
 // its own run plus the number of smaller elements of the other run, by binary search; indices are
@@ -416,15 +390,15 @@ __device__ __forceinline__ uint32_t count_below(const uint32_t* __restrict__ run
     return a;
 }
 
-__global__ __launch_bounds__(kHashBlock) void k_gather_crowded(
-    Scenes S, const uint2* __restrict__ crowd, const uint32_t* __restrict__ crowd_count, uint32_t crowd_cap,
+__device__ __forceinline__ void gather_crowded_cells(
+    const Scenes& S, const uint2* __restrict__ crowd, const uint32_t* __restrict__ crowd_count, uint32_t crowd_cap,
     uint32_t* __restrict__ slot_tmp, const uint2* __restrict__ range, const float4* __restrict__ posrho_in,
     const float4* __restrict__ velp_in, float4* __restrict__ posrho_out, float4* __restrict__ velp_out,
     uint32_t* __restrict__ sorted, uint32_t* __restrict__ skey, const uint32_t* __restrict__ id_in,
-    uint32_t* __restrict__ id_out, uint32_t* __restrict__ div_flags) {
-    __shared__ uint32_t s_v[kCrowdChunk];
+    uint32_t* __restrict__ id_out, uint32_t* __restrict__ div_flags, uint32_t* s_v, uint32_t block,
+    uint32_t nblocks) {
     const uint32_t listed = min(*crowd_count, crowd_cap);
-    for (uint32_t item = blockIdx.x; item < listed; item += gridDim.x) {
+    for (uint32_t item = block; item < listed; item += nblocks) {
         const uint2 e = crowd[item];  // (context cell, scene)
         const SceneDesc& d = S.desc[e.y];
         const uint2 rg = range[e.x];
@@ -478,6 +452,45 @@ __global__ __launch_bounds__(kHashBlock) void k_gather_crowded(
                         div_flags);
         __syncthreads();  // the next cell reuses the LDS chunk
     }
+}
+
+__global__ __launch_bounds__(kHashBlock) void k_gather(Scenes S, uint32_t* __restrict__ slot_tmp,
+                                                       const uint32_t* __restrict__ key,
+                                                       const uint2* __restrict__ range,
+                                                       const float4* __restrict__ posrho_in,
+                                                       const float4* __restrict__ velp_in,
+                                                       float4* __restrict__ posrho_out, float4* __restrict__ velp_out,
+                                                       uint32_t* __restrict__ sorted, uint32_t* __restrict__ skey,
+                                                       const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id_out,
+                                                       uint32_t* __restrict__ div_flags,
+                                                       const uint2* __restrict__ crowd,
+                                                       const uint32_t* __restrict__ crowd_count, uint32_t crowd_cap) {
+    __shared__ uint32_t s_v[kCrowdChunk];
+    if (blockIdx.x < kCrowdBlocks) {  // the leading workgroups take the crowded cells k_place listed
+        gather_crowded_cells(S, crowd, crowd_count, crowd_cap, slot_tmp, range, posrho_in,
+                             velp_in, posrho_out, velp_out, sorted, skey, id_in, id_out, div_flags, s_v, blockIdx.x,
+                             kCrowdBlocks);
+        return;
+    }
+    // kCrowdBlocks is a multiple of the XCD count, so block b still runs on XCD b mod 8
+    const uint32_t b = xcd_block(blockIdx.x - kCrowdBlocks, gridDim.x - kCrowdBlocks);
+    const uint32_t s = scene_of(S.scene256, S.num_scenes, b);
+    const SceneDesc& d = S.desc[s];
+    const uint32_t lt = (b - S.blk256[s]) * kHashBlock + threadIdx.x;
+    if (lt >= d.c.n) return;
+    const uint32_t t = d.p_off + lt;
+    const uint32_t lim = d.p_off + d.c.n;
+    const uint32_t i = slot_tmp[t];
+    if (i - d.p_off >= d.c.n) return;  // only after a failed scan (error flag set)
+    const uint32_t k = key[i];
+    const uint2 rg = range[d.c_off + k];
+    if (rg.y > kGatherDirectMax) return;  // a crowded cell: sorted by the leading workgroups
+    const uint32_t end = min(rg.x + rg.y, lim);
+    uint32_t r = 0;
+    for (uint32_t u = rg.x; u < end; ++u) r += (slot_tmp[u] < i) ? 1u : 0u;
+    const uint32_t dst = rg.x + r;
+    if (dst >= lim) return;
+    move_record(i, dst, k, posrho_in, velp_in, posrho_out, velp_out, sorted, skey, id_in, id_out, div_flags);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1710,20 +1723,16 @@ hipError_t launch_scan(const DeviceBuffers& b, hipStream_t s) {
 hipError_t launch_place(const DeviceBuffers& b, hipStream_t s) {
     if (b.sc.blocks256 == 0) return hipSuccess;
     hipLaunchKernelGGL(k_place, dim3(b.sc.blocks256), dim3(kHashBlock), 0, s, b.sc, b.key, b.rank, b.range,
-                       b.slot_tmp);
+                       b.slot_tmp, b.crowd, b.crowd_count, b.crowd_cap, b.error_flag);
     return hipGetLastError();
 }
 
 hipError_t launch_gather(const DeviceBuffers& b, hipStream_t s) {
     if (b.sc.blocks256 == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather, dim3(b.sc.blocks256), dim3(kHashBlock), 0, s, b.sc, b.slot_tmp, b.key, b.range,
-                       b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, b.ids[0], b.ids[1],
-                       b.rho_max + 1, b.crowd, b.crowd_count, b.crowd_cap, b.error_flag);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gather_crowded, dim3(kCrowdBlocks), dim3(kHashBlock), 0, s, b.sc, b.crowd, b.crowd_count,
-                       b.crowd_cap, b.slot_tmp, b.range, b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted,
-                       b.skey, b.ids[0], b.ids[1], b.rho_max + 1);
+    static_assert(kCrowdBlocks % kNumXcd == 0, "the leading workgroups keep the XCD mapping of the others");
+    hipLaunchKernelGGL(k_gather, dim3(b.sc.blocks256 + kCrowdBlocks), dim3(kHashBlock), 0, s, b.sc, b.slot_tmp, b.key,
+                       b.range, b.posrho[0], b.velp[0], b.posrho[1], b.velp[1], b.sorted, b.skey, b.ids[0],
+                       b.ids[1], b.rho_max + 1, b.crowd, b.crowd_count, b.crowd_cap);
     return hipGetLastError();
 }
 
