@@ -1483,7 +1483,7 @@ __global__ __launch_bounds__(256) void k_closure_select(const float4* __restrict
         const float* f = reinterpret_cast<const float*>(posrho + w.u);
         w.q = float3{f[0], f[1], f[2]};
     };
-    WaveRow a, b, c;
+    WaveRow a{}, b{}, c{};
     bool have_a = next_row(a), have_b = false;
     if (have_a) {
         positions(a);
