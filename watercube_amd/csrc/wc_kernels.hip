@@ -1215,8 +1215,6 @@ __device__ __forceinline__ void select_by_threshold(uint32_t t, uint32_t m, uint
     static_assert(CAP <= (int)kNeighbourListMax && CAP - kClosureNeighbours <= kTop, "one network of 8");
     static_assert(CAP % kGroup == 0 && kGroups >= 2, "whole groups");
     const float4 p = posrho[t];
-    // the list's columns in order, through one advancing scalar pointer (the empty asm keeps the
-    // compiler from holding every column's address in scalar registers)
     auto entries = [&](int g, uint32_t (&u)[kGroup]) {
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) u[j] = load_at(nlist + (size_t)(g * kGroup + j) * stride, t);
