@@ -1,6 +1,7 @@
 """Fixed workload for profiler runs (rocprofv3 kernel traces and PMC counters).
 
-    python3 -m watercube_amd.profile_workload [--workload block_1m|scenes_256] [--steps 20] [--mode graph|eager]
+    python3 -m watercube_amd.profile_workload [--workload block_1m|block_1m_h23|scenes_256] [--steps 20]
+        [--mode graph|eager]
 
 Uploads the workload's initial state (the upload's record-split kernel `k_split` reads 32 bytes
 and writes 32 bytes per particle: a plain copy of known size, used to calibrate the byte
@@ -69,7 +70,12 @@ def main() -> int:
 
     if args.workload == "scenes_256":
         return batch_workload(args)
-    p, parts, dt, _ = scenes.workload(args.workload)
+    if args.workload == "block_1m_h23":
+        # the 1M block's lattice with h = 2.3 spacings: about 50 listed neighbours per particle
+        p, parts = scenes.scaled_block((100, 100, 100), 0.008, seed=3, h_over_s=2.3)
+        dt = 1e-4
+    else:
+        p, parts, dt, _ = scenes.workload(args.workload)
     print(f"profile_workload: {args.workload} n={p.num_particles} copy-calibration bytes/dispatch "
           f"read={32 * p.num_particles} write={32 * p.num_particles}", flush=True)
     with Fluid(p, parts) as f:
